@@ -1,0 +1,120 @@
+/*
+ * st_api.h -- C-ABI of the MI355X per-RK-stage pipeline for the periodic
+ * single-fluid shock-turbulence solver.
+ *
+ * Each entry point replaces one subroutine of the reference Fortran path
+ * (paths relative to the reference repository); the Fortran module
+ * st_mod.f90 binds these with iso_c_binding.
+ *
+ * Arrays are Fortran-ordered F(My,Mz,Mx) (y unit stride), several fields
+ * stacked last: Q(My,Mz,Mx,5) = [rho*u, rho*v, rho*w, rho, rho*E]
+ * (src/main.f:28, src/header:7-10).  Host pointers unless a name ends in
+ * _dev.  All calls return 0 on success, a negative code on failure, and
+ * never abort; st_last_error() gives the message.  Calls on one context
+ * run in order on that context's stream; host-pointer calls return after
+ * their results are on the host.
+ */
+#ifndef ST_API_H
+#define ST_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct st_ctx st_ctx;
+
+/* Grid and run parameters: the runtime form of the compile-time header
+ * (src/header:5-10) and the input deck (src/define_periodic_sf.f:20-80). */
+typedef struct {
+    int nx, ny, nz;
+    double period_x, period_y, period_z;
+    double x_start, y_start, z_start;
+} st_grid;
+
+typedef struct {
+    double Re, Pr, gamma, vis_exp, bulk_ratio;
+    int hyper_param;       /* 1: artificial transport on (src/hyper_rhs.f:50-52) */
+    double cfl_cal;
+    int d2_mode;           /* 0: consistent FDL3DI D2; 1: coefficients as shipped */
+    int rk_scheme;         /* 0: CC5 then LDDRK6; 1: LDDRK5 then LDDRK6; 2: CC5 only */
+} st_params;
+
+/* Setup_Grid + Setup_Derivatives + Init_Filters + Initialize_MPI at one rank
+ * (src/grid.f:15, src/derivCDopt6_periodic.f:9, src/filters_periodic.f:9,
+ * src/initialize_mpi_periodic.f90:1). Allocates all device buffers. */
+int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out);
+void st_ctx_destroy(st_ctx *ctx);
+const char *st_last_error(const st_ctx *ctx);
+int st_sync(st_ctx *ctx);
+
+/* State in / out (the /RKWORK/ Q array, src/main.f:31). */
+int st_state_upload(st_ctx *ctx, const double *Q);
+int st_state_download(st_ctx *ctx, double *Q);
+/* Device address of the resident state Q (5 fields); for callers that keep
+ * data on the GPU. */
+int st_state_dev(st_ctx *ctx, double **Q_dev);
+
+/* DFDX/DFDY/DFDZ(F, dF, iadd, cadd)  src/derivCDopt6_periodic.f:164,400,631.
+ * dir 0,1,2 = x,y,z; iadd 0: dF = cadd*dF/ds, 1: dF += cadd*dF/ds. */
+int st_d1(st_ctx *ctx, int dir, const double *F, double *dF, int iadd, double cadd);
+/* D2FDX2/D2FDY2/D2FDZ2(F, dF, d2F, iadd)  :281,515,747; iadd=1 also sets dF. */
+int st_d2(st_ctx *ctx, int dir, const double *F, double *dF, double *d2F, int iadd);
+/* Compact_Filter(F, F0, N)  src/filters_periodic.f:90 (X, then Y, then Z). */
+int st_filter(st_ctx *ctx, const double *F, double *F0, int nf);
+/* Compact_Filter_X/Y/Z  src/filters_periodic.f:115,254,393. */
+int st_filter_dir(st_ctx *ctx, int dir, const double *F, double *F0);
+/* Gaussian_Filter(F, Fg, N)  src/filters_periodic.f:540. */
+int st_gauss(st_ctx *ctx, const double *F, double *Fg, int nf);
+/* Gaussian_Filter_X/Y/Z  src/filters_periodic.f:565,680,793. */
+int st_gauss_dir(st_ctx *ctx, int dir, const double *F, double *Fg);
+
+/* RHS(Q, Qrhs)  src/rhsv_periodic_sf.f:9.  Q == NULL uses the resident
+ * state; Q is not modified.  Leaves the 12 first derivatives (/GRAD/) on
+ * the device for st_hyper_rhs. Qrhs == NULL keeps the result on the device. */
+int st_rhs(st_ctx *ctx, const double *Q, double *Qrhs);
+/* Hyper_RHS(Q, Qrhs)  src/hyper_rhs.f:12: adds the artificial-transport
+ * terms to the right-hand side of the preceding st_rhs (same Q). */
+int st_hyper_rhs(st_ctx *ctx, double *Qrhs);
+/* Compute_hyper(Q) coefficients after st_hyper_rhs: beta*, mu*, kappa*
+ * (src/hyper_dynamic.f:173, /HYPER/). */
+int st_get_hyper(st_ctx *ctx, double *H3);
+int st_set_hyper(st_ctx *ctx, const double *H3);
+/* /GRAD/ after st_rhs: dux_dx duy_dx duz_dx dux_dy duy_dy duz_dy dux_dz
+ * duy_dz duz_dz dTdx dTdy dTdz (src/rhsv_periodic_sf.f:55-58). */
+int st_get_grad(st_ctx *ctx, double *G12);
+
+/* Fix_Pressure(Q)  src/lddrk4_periodic_sf.f:436 (host array in place). */
+int st_fix_pressure(st_ctx *ctx, double *Q);
+/* Compute_CFL  src/lddrk4_periodic_sf.f:7, on the resident state. */
+int st_cfl(st_ctx *ctx, double *cfl_stab, double *cfl_resv);
+/* Compute_Tstep  src/lddrk4_periodic_sf.f:74 (hyper limits when step > 1). */
+int st_tstep(st_ctx *ctx, double cfl_resv, int step, double *dt);
+/* One stage of LDDRK4 (src/lddrk4_periodic_sf.f:344-356) on the resident
+ * state: RHS, Hyper_RHS, QW = a QW + dt dQ, Q += b QW, filter, Fix_Pressure.
+ * reset_qw = 1 zeroes QW first (start of each stage loop, :339, :376). */
+int st_rk_stage(st_ctx *ctx, double alpha, double beta, double dt, int reset_qw);
+/* LDDRK4(time, step)  src/lddrk4_periodic_sf.f:311 with a given dt. */
+int st_step(st_ctx *ctx, double dt, double *time, int *step);
+/* nsteps of march mode 1 (src/main.f:129-136): Compute_CFL, Compute_Tstep,
+ * LDDRK4 per step.  dt_out (may be NULL) receives each step's dt. */
+int st_march(st_ctx *ctx, int nsteps, double *time, int *step, double *dt_out);
+/* Compute_Mom, Compute_KE, Compute_Mass, Compute_Energy, Compute_Entropy
+ * (src/lddrk4_periodic_sf.f:153-302): MomX MomY MomZ KE Mass Energy Entropy. */
+int st_diagnostics(st_ctx *ctx, double *out7);
+
+/* Per-pass GPU timing with HIP events (0 off, 1 on).  st_profile_read
+ * returns, for up to nmax pass kinds, the launch count, total milliseconds
+ * and the fields read+written per launch; names via st_pass_name. */
+int st_profile(st_ctx *ctx, int on);
+int st_profile_read(st_ctx *ctx, int nmax, int *count, double *total_ms, double *fields_rw);
+const char *st_pass_name(int kind);
+
+/* Seeded isotropic-turbulence initial condition (new; the reference restarts
+ * from a file, src/main.f:69-71).  Q(My,Mz,Mx,5) on the host. */
+int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,
+                    double mt, double gamma, unsigned long long seed, double *Q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,795 @@
+// Host side of the C-ABI (include/st_api.h): context, buffers, pass
+// sequencing for RHS / Hyper_RHS / LDDRK4, reductions and profiling.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/st_api.h"
+#include "st_internal.h"
+#include "st_kernels.h"
+
+namespace {
+
+constexpr int NKIND = 24;          // pass kinds + cross + reductions + fills
+constexpr int KIND_CROSS = 20, KIND_RED = 21, KIND_FILL = 22;
+
+// Fields read + written per point by one launch of each kind (RMW counts
+// twice); the algorithmic bytes of a launch are this times 8 B per point.
+const char *kind_names[NKIND] = {
+    "d1", "d2", "filter", "gauss", "grad", "mix_y", "mix_z", "mixh_x", "mixh_y", "mixh_z",
+    "lap2", "lap2_eta", "gauss_eta", "gauss_coef", "hyper_div", "rk_filter_x", "filter_y",
+    "filter_z_fix", "", "", "cross", "reduce", "fill", ""};
+
+struct Event {
+    hipEvent_t a, b;
+    int kind;
+};
+
+}  // namespace
+
+struct st_ctx {
+    st_grid g;
+    st_params p;
+    int Mx, My, Mz;
+    long N;
+    double sx1[3];
+    st::Phys ph;
+    st::Coef C;
+    st::DirTab th[6];               // [dir] 256-thread layout, [3+dir] 512-thread layout
+    st::DirTab *td = nullptr;
+    double *buf = nullptr;
+    double *Q, *QW, *dQ, *G, *L2e, *S, *Cb, *L2S, *L4e, *L4S, *eta, *Hc, *T5, *tB, *tC;
+    double *red = nullptr;          // reduction partials (device)
+    double *red_h = nullptr;        // pinned host copy
+    hipStream_t s = nullptr;
+    char err[512];
+    int prof = 0;
+    std::vector<Event> ev;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[NKIND];
+    long prof_cnt[NKIND];
+    double prof_fields[NKIND];
+};
+
+namespace {
+
+int fail(st_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) return fail(c, -3, "%s: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+
+hipEvent_t ev_get(st_ctx *c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_begin(st_ctx *c, int kind, Event &e)
+{
+    e.kind = kind;
+    e.a = ev_get(c);
+    e.b = ev_get(c);
+    hipEventRecord(e.a, c->s);
+}
+
+void prof_end(st_ctx *c, Event &e)
+{
+    hipEventRecord(e.b, c->s);
+    c->ev.push_back(e);
+}
+
+// Collect finished events (after a synchronisation).
+void prof_collect(st_ctx *c)
+{
+    for (auto &e : c->ev) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            c->prof_ms[e.kind] += ms;
+            c->prof_cnt[e.kind] += 1;
+        }
+        c->pool.push_back(e.a);
+        c->pool.push_back(e.b);
+    }
+    c->ev.clear();
+}
+
+st::PassArgs base_args(st_ctx *c)
+{
+    st::PassArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.C = c->C;
+    a.ph = c->ph;
+    a.N = c->N;
+    a.Q = c->Q;
+    a.QW = c->QW;
+    a.dQ = c->dQ;
+    a.G = c->G;
+    a.L2e = c->p.hyper_param ? c->L2e : nullptr;
+    a.S = c->S;
+    a.Cb = c->Cb;
+    a.L2S = c->L2S;
+    a.L4e = c->L4e;
+    a.L4S = c->L4S;
+    a.eta = c->eta;
+    a.Hc = c->Hc;
+    return a;
+}
+
+// fields read+written per point for one launch of a pass kind
+double kind_fields(int kind, const st::PassArgs &a)
+{
+    switch (kind) {
+    case st::K_D1: return a.iadd ? 3 : 2;
+    case st::K_D2: return a.iadd ? 3 : 2;
+    case st::K_FILT: return 2;
+    case st::K_GAUSS: return 2;
+    case st::K_GRAD: return 5 + 4 + (a.first ? 5 : 10) + (a.L2e ? (a.first ? 1 : 2) : 0);
+    case st::K_MIX_Y: return 4 + 5 + 8;
+    case st::K_MIX_Z: return 2 + 5 + 6;
+    case st::K_MIXH_X: return 2 + 2;
+    case st::K_MIXH_Y: return 6 + 5 + 8 + 4;
+    case st::K_MIXH_Z: return 4 + 5 + 6 + 4;
+    case st::K_LAP2: return a.first ? 2 : 3;
+    case st::K_LAP2_LAST: return 1 + 1 + 1 + 5 + 2;
+    case st::K_GAUSS_ETA: return 4;
+    case st::K_GAUSS_ETA_LAST: return 2 + 1 + 3;
+    case st::K_HDIV: return 3 + 8 + 4 + 8;
+    case st::K_RKF_X: return 15 + 5 + 5;
+    case st::K_F_Y: return 10;
+    case st::K_F_Z: return 10;
+    case KIND_CROSS: return 12 + 5 + 8 + 2;
+    default: return 0;
+    }
+}
+
+int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
+{
+    const int lay = st::pass_narrow(kind) ? 3 : 0;
+    const st::DirTab &t = c->th[lay + dir];
+    a.tab = c->td + lay + dir;
+    a.K1 = cadd * c->sx1[dir] * (double)(t.n - 1) * t.op[0].scale;   // fac of src/derivCDopt6_periodic.f:181
+    a.K2 = t.s2 * t.op[1].scale;
+    Event e;
+    if (c->prof) prof_begin(c, kind, e);
+    hipError_t err = st::run_pass(kind, t, dir, c->My, c->Mz, c->Mx, a, c->s);
+    if (c->prof) {
+        prof_end(c, e);
+        c->prof_fields[kind] = kind_fields(kind, a);
+    }
+    if (err != hipSuccess) return fail(c, -4, "pass %s dir %d: %s", kind_names[kind], dir, hipGetErrorString(err));
+    return 0;
+}
+
+}  // namespace
+
+
+// ---------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------
+static double leng_of(double period, int n) { return period * (double)(n - 1) / (double)n; }
+
+extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out)
+{
+    if (!grid || !params || !out) return -1;
+    *out = nullptr;
+    st_ctx *c = new st_ctx();
+    c->err[0] = 0;
+    c->g = *grid;
+    c->p = *params;
+    c->Mx = grid->nx;
+    c->My = grid->ny;
+    c->Mz = grid->nz;
+    c->N = (long)c->Mx * c->My * c->Mz;
+    std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
+    std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
+    std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
+    const int n[3] = {c->Mx, c->My, c->Mz};
+    const double period[3] = {grid->period_x, grid->period_y, grid->period_z};
+    const double start[3] = {grid->x_start, grid->y_start, grid->z_start};
+    for (int d = 0; d < 3; d++) {
+        int M, P;
+        if (n[d] < 16 || !st::choose_layout(n[d], &M, &P)) {
+            fail(c, -2, "unsupported line length %d in direction %d (need n = M*P, M in {2,4,6,8}, P in {8,16,32,64})", n[d], d);
+            *out = c;
+            return -2;
+        }
+        const int B = 256 / P;             // lines per tile (same for the 512-thread layout)
+        const long lot = c->N / n[d];
+        if (lot % B) {
+            fail(c, -2, "line count %ld of direction %d not a multiple of the tile", lot, d);
+            *out = c;
+            return -2;
+        }
+    }
+    // physical constants
+    const double gamma = params->gamma;
+    c->ph.gamma = gamma;
+    c->ph.gm1 = gamma - 1.0;
+    c->ph.gfac = gamma / (gamma - 1.0);
+    c->ph.gm1_inv = 1.0 / (gamma - 1.0);
+    c->ph.ginv = 1.0 / gamma;
+    c->ph.re_inv = params->Re > 0.0 ? 1.0 / params->Re : 0.0;
+    c->ph.pr_inv = params->Pr > 0.0 ? 1.0 / params->Pr : 0.0;
+    c->ph.vis_exp = params->vis_exp;
+    c->ph.vis_exp_m1 = params->vis_exp - 1.0;
+    c->ph.bulk = params->bulk_ratio;
+    c->ph.cmu = 2.0e-3;
+    c->ph.ck = 1.0e-2;
+    c->ph.cfl_cal = params->cfl_cal;
+    for (int d = 0; d < 3; d++) {
+        const double leng = leng_of(period[d], n[d]);
+        const double g1 = start[d] + leng * 0.0 / (double)(n[d] - 1);
+        const double g2 = start[d] + leng * 1.0 / (double)(n[d] - 1);
+        c->ph.D[d] = g2 - g1;                                  // src/grid.f:40
+    }
+    c->ph.dx6 = std::pow(c->ph.D[0], 6.0);
+    c->ph.dx5 = std::pow(c->ph.D[0], 5.0);
+    // scheme coefficients (src/header:193-203, src/LES_header:107-113,153-157)
+    st::Coef &C = c->C;
+    C.a1 = 14.0 / 18.0;
+    C.b1 = 1.0 / 36.0;
+    C.c1 = 0.0;
+    C.a2 = 12.0 / 11.0;
+    C.b2 = 3.0 / 44.0;
+    if (params->d2_mode == 1) {
+        C.c2 = 2.0 * (C.a2 + C.b2);
+        C.d2 = 0.0;
+    } else {
+        C.c2 = 0.0;
+        C.d2 = 2.0 * (C.a2 + C.b2);
+    }
+    C.f0 = 9.9965e-01;
+    C.f1 = 0.66652e+00;
+    C.f2 = 0.16674e+00;
+    C.f3 = 4.0e-05;
+    C.f4 = -5.0e-06;
+    C.g0 = 3565.0 / 10368.0;
+    C.g1 = 3091.0 / 12960.0;
+    C.g2 = 1997.0 / 25920.0;
+    C.g3 = 149.0 / 12960.0;
+    C.g4 = 107.0 / 103680.0;
+    for (int d = 0; d < 3; d++) {
+        int M, P;
+        st::choose_layout(n[d], &M, &P);
+        const double sx1 = 1.0 / leng_of(period[d], n[d]);   // src/grid.f:31-32
+        const double sx2 = sx1 * sx1;
+        c->sx1[d] = sx1;
+        if (st::build_dirtab(&c->th[d], n[d], M, P, 256 / P, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0, 6.6624e-01, 1.6688e-01) ||
+            (M % 2 == 0 && 2 * P <= 64 &&
+             st::build_dirtab(&c->th[3 + d], n[d], M / 2, 2 * P, 512 / (2 * P), sx1, sx2, 1.0 / 3.0, 2.0 / 11.0,
+                              6.6624e-01, 1.6688e-01))) {
+            fail(c, -2, "coefficient tables failed for direction %d", d);
+            *out = c;
+            return -2;
+        }
+        if (M % 2 || 2 * P > 64) {
+            fail(c, -2, "line length %d in direction %d has no 512-thread layout", n[d], d);
+            *out = c;
+            return -2;
+        }
+    }
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)) != hipSuccess) {
+        fail(c, -3, "hipStreamCreate: %s", hipGetErrorString(e));
+        *out = c;
+        return -3;
+    }
+    const long nf = 5 + 5 + 5 + 12 + 1 + 1 + 1 + 1 + 1 + 1 + 2 + 3 + 5 + 1 + 1;
+    if ((e = hipMalloc(&c->buf, sizeof(double) * nf * c->N)) != hipSuccess ||
+        (e = hipMalloc(&c->td, sizeof(st::DirTab) * 6)) != hipSuccess ||
+        (e = hipMalloc(&c->red, sizeof(double) * 1024 * 8)) != hipSuccess ||
+        (e = hipHostMalloc(&c->red_h, sizeof(double) * 1024 * 8)) != hipSuccess) {
+        fail(c, -3, "device allocation: %s", hipGetErrorString(e));
+        *out = c;
+        return -3;
+    }
+    double *p = c->buf;
+    const long N = c->N;
+    c->Q = p; p += 5 * N;
+    c->QW = p; p += 5 * N;
+    c->dQ = p; p += 5 * N;
+    c->G = p; p += 12 * N;
+    c->L2e = p; p += N;
+    c->S = p; p += N;
+    c->Cb = p; p += N;
+    c->L2S = p; p += N;
+    c->L4e = p; p += N;
+    c->L4S = p; p += N;
+    c->eta = p; p += 2 * N;
+    c->Hc = p; p += 3 * N;
+    c->T5 = p; p += 5 * N;
+    c->tB = p; p += N;
+    c->tC = p; p += N;
+    if ((e = hipMemcpyAsync(c->td, c->th, sizeof(st::DirTab) * 6, hipMemcpyHostToDevice, c->s)) != hipSuccess ||
+        (e = st::run_fill(c->buf, nf * N, 0.0, c->s)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->s)) != hipSuccess) {
+        fail(c, -3, "device init: %s", hipGetErrorString(e));
+        *out = c;
+        return -3;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" void st_ctx_destroy(st_ctx *c)
+{
+    if (!c) return;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (auto &e : c->ev) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    for (auto e : c->pool) hipEventDestroy(e);
+    if (c->buf) hipFree(c->buf);
+    if (c->td) hipFree(c->td);
+    if (c->red) hipFree(c->red);
+    if (c->red_h) hipHostFree(c->red_h);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+}
+
+extern "C" const char *st_last_error(const st_ctx *c) { return c ? c->err : "null context"; }
+
+extern "C" int st_sync(st_ctx *c)
+{
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------
+namespace {
+
+// RK coefficient tables: CC5 (src/header:284-296), LDDRK6 (src/header:267-281,
+// identical to init/header:222-236), LDDRK5 (src/header:253-265).
+const double rk_cc5_a[5] = {0.0, -4.801594388477952e-01, -1.404247195200064e+00, -2.016477077503357e+00, -1.056444269767207e+00};
+const double rk_cc5_b[5] = {1.028639988104959e-01, 7.408540575766301e-01, 7.426530946683799e-01, 4.694937902357905e-01, 1.881733382887932e-01};
+const double rk_ld6_a[6] = {0.0, -4.4127377153877382565e-1, -1.073982008079781868e0, -1.7063570791256758809e0,
+                            -2.7979293162682443056e0, -4.0913537120919160454e0};
+const double rk_ld6_b[6] = {1.1584888181285561688e-1, 3.7287699051652864918e-1, 7.3795368921435295698e-1,
+                            5.7981109366311039538e-1, 1.031284991300145194e0, 1.5e-1};
+const double rk_ld5_a[5] = {0.0, -6.0512264332862261228e-1, -2.04375640234761394333e0, -0.74069990637544192841e0,
+                            -4.42317651302968168941e0};
+const double rk_ld5_b[5] = {2.6874543888713438496e-1, 8.014706973220802933e-1, 5.0515704269422722538e-1,
+                            5.6235680379000296407e-1, 5.900655127758823335e-2};
+
+int rk_loop(int scheme, int loop, const double **a, const double **b)
+{
+    if (loop == 0) {
+        if (scheme == 1) { *a = rk_ld5_a; *b = rk_ld5_b; return 5; }
+        *a = rk_cc5_a; *b = rk_cc5_b; return 5;
+    }
+    if (loop == 1 && scheme != 2) { *a = rk_ld6_a; *b = rk_ld6_b; return 6; }
+    return 0;
+}
+
+int up(st_ctx *c, double *dev, const double *host, long nfields)
+{
+    HIPCHK(c, hipMemcpyAsync(dev, host, sizeof(double) * nfields * c->N, hipMemcpyHostToDevice, c->s));
+    return 0;
+}
+
+int down(st_ctx *c, double *host, const double *dev, long nfields)
+{
+    HIPCHK(c, hipMemcpyAsync(host, dev, sizeof(double) * nfields * c->N, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    return 0;
+}
+
+int fill(st_ctx *c, double *p, long n, double v)
+{
+    Event e;
+    if (c->prof) prof_begin(c, KIND_FILL, e);
+    hipError_t err = st::run_fill(p, n, v, c->s);
+    if (c->prof) prof_end(c, e);
+    if (err != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(err));
+    return 0;
+}
+
+#define TRY(x)                \
+    do {                      \
+        int r_ = (x);         \
+        if (r_) return r_;    \
+    } while (0)
+
+// RHS (src/rhsv_periodic_sf.f:9-25) into dQ; with the artificial transport
+// on, also the strain magnitude, sensor and first Laplacians it needs.
+int seq_rhs(st_ctx *c)
+{
+    const int hyp = c->p.hyper_param;
+    st::PassArgs a = base_args(c);
+    a.first = 1;
+    TRY(pass(c, st::K_GRAD, 1, a));
+    a.first = 0;
+    TRY(pass(c, st::K_GRAD, 2, a));
+    TRY(pass(c, st::K_GRAD, 0, a));
+    {
+        Event e;
+        if (c->prof) prof_begin(c, KIND_CROSS, e);
+        hipError_t err = st::run_cross(a, hyp, c->s);
+        if (c->prof) {
+            prof_end(c, e);
+            c->prof_fields[KIND_CROSS] = kind_fields(KIND_CROSS, a);
+        }
+        if (err != hipSuccess) return fail(c, -4, "cross: %s", hipGetErrorString(err));
+    }
+    if (hyp) {
+        a.first = 1;
+        TRY(pass(c, st::K_MIXH_X, 0, a));
+        a.first = 0;
+        TRY(pass(c, st::K_MIXH_Y, 1, a));
+        TRY(pass(c, st::K_MIXH_Z, 2, a));
+    } else {
+        TRY(pass(c, st::K_MIX_Y, 1, a));
+        TRY(pass(c, st::K_MIX_Z, 2, a));
+    }
+    return 0;
+}
+
+// Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ.
+int seq_hyper(st_ctx *c)
+{
+    if (!c->p.hyper_param) return 0;
+    st::PassArgs a = base_args(c);
+    a.first = 1;
+    TRY(pass(c, st::K_LAP2, 0, a));
+    a.first = 0;
+    TRY(pass(c, st::K_LAP2, 1, a));
+    TRY(pass(c, st::K_LAP2_LAST, 2, a));
+    TRY(pass(c, st::K_GAUSS_ETA, 0, a));
+    TRY(pass(c, st::K_GAUSS_ETA, 1, a));
+    TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));
+    TRY(pass(c, st::K_HDIV, 1, a));
+    TRY(pass(c, st::K_HDIV, 2, a));
+    TRY(pass(c, st::K_HDIV, 0, a));
+    return 0;
+}
+
+// One stage body (src/lddrk4_periodic_sf.f:344-356).
+int seq_stage(st_ctx *c, double ra, double rb, double dt)
+{
+    TRY(seq_rhs(c));
+    TRY(seq_hyper(c));
+    st::PassArgs a = base_args(c);
+    a.rk_a = ra;
+    a.rk_b = rb;
+    a.dt = dt;
+    TRY(pass(c, st::K_RKF_X, 0, a));
+    TRY(pass(c, st::K_F_Y, 1, a));
+    a.fix = 1;
+    TRY(pass(c, st::K_F_Z, 2, a));
+    return 0;
+}
+
+int reduce(st_ctx *c, int which, int nv, double *out)
+{
+    st::PassArgs a = base_args(c);
+    int nb = 0;
+    Event e;
+    if (c->prof) prof_begin(c, KIND_RED, e);
+    hipError_t err = st::run_reduce(which, a, c->red, &nb, c->s);
+    if (c->prof) prof_end(c, e);
+    if (err != hipSuccess) return fail(c, -4, "reduce: %s", hipGetErrorString(err));
+    HIPCHK(c, hipMemcpyAsync(c->red_h, c->red, sizeof(double) * nb * nv, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    for (int i = 0; i < nv; i++) {
+        double v = c->red_h[i];
+        for (int b = 1; b < nb; b++) {
+            const double x = c->red_h[b * nv + i];
+            if (which == 0) v = x > v ? x : v;
+            else if (which == 1) v = x < v ? x : v;
+            else v += x;
+        }
+        out[i] = v;
+    }
+    return 0;
+}
+
+int stage_loop(st_ctx *c, double dt, double *time, int *step)
+{
+    for (int loop = 0; loop < 2; loop++) {
+        const double *ra, *rb;
+        const int ns = rk_loop(c->p.rk_scheme, loop, &ra, &rb);
+        if (!ns) break;
+        TRY(fill(c, c->QW, 5 * c->N, 0.0));                  // QW = 0 (:339, :376)
+        for (int k = 0; k < ns; k++) TRY(seq_stage(c, ra[k], rb[k], dt));
+        if (time) *time = *time + dt;                        // :372, :419
+    }
+    if (step) *step = *step + 1;                             // :420
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------
+extern "C" int st_state_upload(st_ctx *c, const double *Q)
+{
+    if (!c || !Q) return -1;
+    TRY(up(c, c->Q, Q, 5));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int st_state_download(st_ctx *c, double *Q)
+{
+    if (!c || !Q) return -1;
+    return down(c, Q, c->Q, 5);
+}
+
+extern "C" int st_state_dev(st_ctx *c, double **Q_dev)
+{
+    if (!c || !Q_dev) return -1;
+    *Q_dev = c->Q;
+    return 0;
+}
+
+extern "C" int st_d1(st_ctx *c, int dir, const double *F, double *dF, int iadd, double cadd)
+{
+    if (!c || !F || !dF || dir < 0 || dir > 2) return fail(c, -1, "st_d1: bad arguments");
+    TRY(up(c, c->T5, F, 1));
+    if (iadd) TRY(up(c, c->tB, dF, 1));
+    st::PassArgs a = base_args(c);
+    a.in0 = c->T5;
+    a.out0 = c->tB;
+    a.iadd = iadd ? 1 : 0;
+    TRY(pass(c, st::K_D1, dir, a, cadd));
+    return down(c, dF, c->tB, 1);
+}
+
+extern "C" int st_d2(st_ctx *c, int dir, const double *F, double *dF, double *d2F, int iadd)
+{
+    if (!c || !F || !d2F || dir < 0 || dir > 2 || (iadd && !dF)) return fail(c, -1, "st_d2: bad arguments");
+    TRY(up(c, c->T5, F, 1));
+    st::PassArgs a = base_args(c);
+    a.in0 = c->T5;
+    a.out0 = c->tB;
+    a.out1 = c->tC;
+    a.iadd = iadd ? 1 : 0;
+    TRY(pass(c, st::K_D2, dir, a));
+    if (iadd) TRY(down(c, dF, c->tB, 1));
+    return down(c, d2F, c->tC, 1);
+}
+
+extern "C" int st_filter_dir(st_ctx *c, int dir, const double *F, double *F0)
+{
+    if (!c || !F || !F0 || dir < 0 || dir > 2) return fail(c, -1, "st_filter_dir: bad arguments");
+    TRY(up(c, c->T5, F, 1));
+    st::PassArgs a = base_args(c);
+    a.in0 = c->T5;
+    a.out0 = c->tB;
+    TRY(pass(c, st::K_FILT, dir, a));
+    return down(c, F0, c->tB, 1);
+}
+
+extern "C" int st_gauss_dir(st_ctx *c, int dir, const double *F, double *Fg)
+{
+    if (!c || !F || !Fg || dir < 0 || dir > 2) return fail(c, -1, "st_gauss_dir: bad arguments");
+    TRY(up(c, c->T5, F, 1));
+    st::PassArgs a = base_args(c);
+    a.in0 = c->T5;
+    a.out0 = c->tB;
+    TRY(pass(c, st::K_GAUSS, dir, a));
+    return down(c, Fg, c->tB, 1);
+}
+
+static int chain3(st_ctx *c, int kind, const double *F, double *F0, int nf)
+{
+    for (int L = 0; L < nf; L++) {
+        TRY(up(c, c->tB, F + (size_t)L * c->N, 1));
+        st::PassArgs a = base_args(c);
+        a.in0 = c->tB;
+        a.out0 = c->tC;
+        TRY(pass(c, kind, 0, a));
+        a.in0 = c->tC;
+        a.out0 = c->tB;
+        TRY(pass(c, kind, 1, a));
+        a.in0 = c->tB;
+        a.out0 = c->tC;
+        TRY(pass(c, kind, 2, a));
+        TRY(down(c, F0 + (size_t)L * c->N, c->tC, 1));
+    }
+    return 0;
+}
+
+extern "C" int st_filter(st_ctx *c, const double *F, double *F0, int nf)
+{
+    if (!c || !F || !F0 || nf < 1) return fail(c, -1, "st_filter: bad arguments");
+    return chain3(c, st::K_FILT, F, F0, nf);
+}
+
+extern "C" int st_gauss(st_ctx *c, const double *F, double *Fg, int nf)
+{
+    if (!c || !F || !Fg || nf < 1) return fail(c, -1, "st_gauss: bad arguments");
+    return chain3(c, st::K_GAUSS, F, Fg, nf);
+}
+
+extern "C" int st_rhs(st_ctx *c, const double *Q, double *Qrhs)
+{
+    if (!c) return -1;
+    if (Q) TRY(up(c, c->Q, Q, 5));
+    TRY(seq_rhs(c));
+    if (Qrhs) return down(c, Qrhs, c->dQ, 5);
+    return 0;
+}
+
+extern "C" int st_hyper_rhs(st_ctx *c, double *Qrhs)
+{
+    if (!c) return -1;
+    if (!c->p.hyper_param) return 0;                         // src/hyper_rhs.f:50-52
+    if (Qrhs) TRY(up(c, c->dQ, Qrhs, 5));
+    TRY(seq_hyper(c));
+    if (Qrhs) return down(c, Qrhs, c->dQ, 5);
+    return 0;
+}
+
+extern "C" int st_get_hyper(st_ctx *c, double *H3)
+{
+    if (!c || !H3) return -1;
+    return down(c, H3, c->Hc, 3);
+}
+
+extern "C" int st_set_hyper(st_ctx *c, const double *H3)
+{
+    if (!c || !H3) return -1;
+    TRY(up(c, c->Hc, H3, 3));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int st_get_grad(st_ctx *c, double *G12)
+{
+    if (!c || !G12) return -1;
+    return down(c, G12, c->G, 12);
+}
+
+extern "C" int st_fix_pressure(st_ctx *c, double *Q)
+{
+    if (!c || !Q) return -1;
+    TRY(up(c, c->T5, Q, 5));
+    st::PassArgs a = base_args(c);
+    hipError_t err = st::run_fixp(a, c->T5, c->s);
+    if (err != hipSuccess) return fail(c, -4, "fix_pressure: %s", hipGetErrorString(err));
+    return down(c, Q, c->T5, 5);
+}
+
+extern "C" int st_cfl(st_ctx *c, double *stab, double *resv)
+{
+    if (!c) return -1;
+    double v[2];
+    TRY(reduce(c, 0, 2, v));
+    if (stab) *stab = v[0];
+    if (resv) *resv = v[1];
+    return 0;
+}
+
+extern "C" int st_tstep(st_ctx *c, double cfl_resv, int step, double *dt)
+{
+    if (!c || !dt) return -1;
+    double t = c->p.cfl_cal / cfl_resv;                      // src/lddrk4_periodic_sf.f:96
+    if (c->p.hyper_param == 1 && step > 1) {                  // :99
+        double v[3];
+        TRY(reduce(c, 1, 3, v));
+        const double p2 = (double)0.2f;                       // default-real literal 0.2 (:141)
+        if (p2 * v[0] < t) t = p2 * v[0];
+        if (p2 * v[1] < t) t = p2 * v[1];
+        if (p2 * v[2] < t) t = p2 * v[2];
+    }
+    *dt = t;
+    return 0;
+}
+
+extern "C" int st_rk_stage(st_ctx *c, double alpha, double beta, double dt, int reset_qw)
+{
+    if (!c) return -1;
+    if (reset_qw) TRY(fill(c, c->QW, 5 * c->N, 0.0));
+    TRY(seq_stage(c, alpha, beta, dt));
+    return 0;
+}
+
+extern "C" int st_step(st_ctx *c, double dt, double *time, int *step)
+{
+    if (!c) return -1;
+    TRY(stage_loop(c, dt, time, step));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    return 0;
+}
+
+extern "C" int st_march(st_ctx *c, int nsteps, double *time, int *step, double *dt_out)
+{
+    if (!c || !time || !step) return -1;
+    for (int it = 0; it < nsteps; it++) {
+        double stab, resv, dt;
+        TRY(st_cfl(c, &stab, &resv));
+        TRY(st_tstep(c, resv, *step, &dt));
+        if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, -5, "non-finite or non-positive dt %g at step %d", dt, *step);
+        TRY(stage_loop(c, dt, time, step));
+        if (dt_out) dt_out[it] = dt;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    return 0;
+}
+
+extern "C" int st_diagnostics(st_ctx *c, double *out7)
+{
+    if (!c || !out7) return -1;
+    double v[7];
+    TRY(reduce(c, 2, 7, v));
+    const double NN = (double)(c->Mx * c->My * c->Mz);
+    out7[0] = v[0] / NN;
+    out7[1] = v[1] / NN;
+    out7[2] = v[2] / NN;
+    out7[3] = 0.5 * v[3] / NN;
+    out7[4] = v[4] / NN;
+    out7[5] = v[5] / NN;
+    out7[6] = v[6] / NN;
+    return 0;
+}
+
+extern "C" int st_profile(st_ctx *c, int on)
+{
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    prof_collect(c);
+    c->prof = on ? 1 : 0;
+    std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
+    std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
+    return 0;
+}
+
+extern "C" int st_profile_read(st_ctx *c, int nmax, int *count, double *total_ms, double *fields_rw)
+{
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    prof_collect(c);
+    const int n = nmax < NKIND ? nmax : NKIND;
+    for (int i = 0; i < n; i++) {
+        if (count) count[i] = (int)c->prof_cnt[i];
+        if (total_ms) total_ms[i] = c->prof_ms[i];
+        if (fields_rw) fields_rw[i] = c->prof_fields[i];
+    }
+    return n;
+}
+
+extern "C" const char *st_pass_name(int kind)
+{
+    if (kind < 0 || kind >= NKIND) return "";
+    return kind_names[kind];
+}
+
+extern "C" int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,
+                               double mt, double gamma, unsigned long long seed, double *Q)
+{
+    if (!Q || !period) return -1;
+    return st::ic_isotropic(nx, ny, nz, period, k0, urms, mt, gamma, seed, Q);
+}

@@ -1,0 +1,68 @@
+// Internal declarations shared by the HIP kernels and the host side of the
+// C-ABI.  Nothing here crosses the library boundary.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace st {
+
+// Cyclic constant-band solve of (beta, alpha, 1, alpha, beta) as a product of
+// a forward and a backward linear recurrence.  Order 1 (beta = 0):
+//   A = (1/(1+rho^2)) (1 - rho S)(1 - rho S^-1),  S = cyclic shift,
+// order 2: A = kappa (1 - a1 S - a2 S^2)(1 - a1 S^-1 - a2 S^-2).
+// Each lane of a wave owns M consecutive points of a line; P lanes own a
+// line of n = M*P points.  Chunk-local recurrences are stitched together by
+// a segmented scan over the P lanes.  With the companion matrix
+// A = [[a1, a2], [1, 0]] (order 1: a2 = 0) the tables hold:
+struct OpTab {
+    int order;            // 1 or 2
+    int pad_;
+    double a1, a2;        // recurrence coefficients
+    double scale;         // folded into the right-hand side: 1+rho^2 or 1/kappa
+    double cf[16][2];     // row 0 of A^(k+1), k = 0..M-1 (in-chunk carry weights)
+    double sc[6][4];      // A^(M*2^j), j = 0..log2(P)-1 (scan steps, row-major 2x2)
+    double wrap[4];       // (I - A^n)^-1
+    double lp[64][4];     // A^(M*c), c = 0..P-1 (per-lane injection of the cyclic total)
+};
+
+// Per-direction operator tables and stencil factors.
+struct DirTab {
+    OpTab op[3];          // 0: first derivative, 1: second derivative, 2: compact filter
+    double s1;            // sx(1)*(N-1)        (src/derivCDopt6_periodic.f:181)
+    double s2;            // (N-1)^2*sx(2)      (src/derivCDopt6_periodic.f:298)
+    int n, M, P, B;       // line length, points per lane, lanes per line, lines per tile
+};
+
+// Compact-scheme coefficients (src/header:191-203, src/LES_header:104-157).
+struct Coef {
+    double a1, b1, c1;            // D1 right-hand side
+    double a2, b2, c2, d2;        // D2 right-hand side (+-1, +-2, +-3, centre)
+    double f0, f1, f2, f3, f4;    // compact filter right-hand side
+    double g0, g1, g2, g3, g4;    // Gaussian
+};
+
+// Physical parameters used in kernels.
+struct Phys {
+    double gamma, gm1, gfac, gm1_inv;   // gamma, gamma-1, gamma/(gamma-1), 1/(gamma-1)
+    double ginv;                        // 1/gamma
+    double re_inv, pr_inv, vis_exp, vis_exp_m1, bulk;
+    double dx6, dx5;                    // DX**6, DX**5 (src/hyper_dynamic.f:307-311)
+    double cmu, ck;                     // C_mu = 2e-3, C_k = 1e-2
+    double D[3];                        // DX, DY, DZ
+    double cfl_cal;
+};
+
+// Fill the three operator tables of one direction for a line of n points
+// split into P lanes of M points.  Returns 0 on success.
+int build_dirtab(DirTab *t, int n, int M, int P, int B, double sx1, double sx2,
+                 double d1_alpha, double d2_alpha, double f_alpha, double f_beta);
+
+// Choose (M, P) for a line length; returns 0 if n is unsupported.
+int choose_layout(int n, int *M, int *P);
+
+// Seeded isotropic turbulence initial condition (host).
+int ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,
+                 double mt, double gamma, unsigned long long seed, double *Q);
+
+}  // namespace st

@@ -1,0 +1,53 @@
+// Kernel-launch interface between the C-ABI host code and the HIP passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "st_internal.h"
+
+namespace st {
+
+// Everything a pass may read or write.  Unused pointers stay null.
+struct PassArgs {
+    const DirTab *tab;    // device copy of this direction's tables
+    Coef C;
+    Phys ph;
+    double K1, K2;        // D1 / D2 right-hand-side scale (stencil factor x solve scale)
+    long N;               // points per field
+    double *Q, *QW, *dQ;  // state, RK register, right-hand side (5 fields each)
+    double *G;            // 12 first derivatives, G[(3*d + i)*N] = du_i/dx_d, G[(9+d)*N] = dT/dx_d
+    double *L2e, *S, *Cb, *L2S, *L4e, *L4S;
+    double *eta;          // 2 fields
+    double *Hc;           // beta*, mu*, kappa*
+    const double *in0;    // single-operator input
+    double *out0, *out1;  // single-operator outputs
+    int iadd;             // single-operator accumulate flag
+    int first;            // accumulators: 1 = overwrite
+    int fix;              // z filter: apply the pressure fix
+    int pad_;
+    double rk_a, rk_b, dt;
+};
+
+enum PassKind {
+    K_D1, K_D2, K_FILT, K_GAUSS,
+    K_GRAD,
+    K_MIX_Y, K_MIX_Z, K_MIXH_X, K_MIXH_Y, K_MIXH_Z,
+    K_LAP2, K_LAP2_LAST,
+    K_GAUSS_ETA, K_GAUSS_ETA_LAST,
+    K_HDIV,
+    K_RKF_X, K_F_Y, K_F_Z,
+};
+
+// true for passes launched with 512 threads (tables built for M/2, 2P)
+bool pass_narrow(int kind);
+hipError_t run_pass(int kind, const DirTab &t, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s);
+
+}  // namespace st
+
+namespace st {
+// which: 0 = CFL (2 maxima), 1 = dt limits (3 minima), 2 = diagnostics (7 sums)
+hipError_t run_reduce(int which, const PassArgs &a, double *partial, int *nblocks, hipStream_t s);
+hipError_t run_fill(double *p, long n, double v, hipStream_t s);
+hipError_t run_fixp(const PassArgs &a, double *Qx, hipStream_t s);
+}  // namespace st

@@ -1,0 +1,297 @@
+"""Python host binding of the MI355X per-stage pipeline (ctypes over libst.so).
+
+The classes mirror the reference's subroutine interface for this path
+(src/lddrk4_periodic_sf.f, src/rhsv_periodic_sf.f, src/hyper_rhs.f,
+src/derivCDopt6_periodic.f, src/filters_periodic.f): DFDX/DFDY/DFDZ,
+D2FDX2/..., Compact_Filter, Gaussian_Filter, RHS, Hyper_RHS, Fix_Pressure,
+Compute_CFL, Compute_Tstep, LDDRK4.  Arrays are numpy float64 in Fortran
+order, (My, Mz, Mx) per field and (My, Mz, Mx, 5) for the state.
+
+There is no CPU fallback: if the HIP library is missing or a call fails,
+an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+LIB_PATH = os.environ.get("ST_AMD_LIB", os.path.join(_PKG, "libst.so"))
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class StError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("period_x", ctypes.c_double), ("period_y", ctypes.c_double), ("period_z", ctypes.c_double),
+                ("x_start", ctypes.c_double), ("y_start", ctypes.c_double), ("z_start", ctypes.c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("Re", ctypes.c_double), ("Pr", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("vis_exp", ctypes.c_double), ("bulk_ratio", ctypes.c_double),
+                ("hyper_param", ctypes.c_int), ("cfl_cal", ctypes.c_double),
+                ("d2_mode", ctypes.c_int), ("rk_scheme", ctypes.c_int)]
+
+
+EXPORTS = [
+    "st_ctx_create", "st_ctx_destroy", "st_last_error", "st_sync", "st_state_upload", "st_state_download",
+    "st_state_dev", "st_d1", "st_d2", "st_filter", "st_filter_dir", "st_gauss", "st_gauss_dir", "st_rhs",
+    "st_hyper_rhs", "st_get_hyper", "st_set_hyper", "st_get_grad", "st_fix_pressure", "st_cfl", "st_tstep",
+    "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
+    "st_ic_isotropic",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libst.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise StError("HIP library not built: %s (run __graft_entry__.build())" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.st_ctx_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.st_ctx_destroy.argtypes = [vp]
+    L.st_ctx_destroy.restype = None
+    L.st_last_error.argtypes = [vp]
+    L.st_last_error.restype = ctypes.c_char_p
+    L.st_sync.argtypes = [vp]
+    L.st_state_upload.argtypes = [vp, _dp]
+    L.st_state_download.argtypes = [vp, _dp]
+    L.st_state_dev.argtypes = [vp, ctypes.POINTER(_dp)]
+    L.st_d1.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double]
+    L.st_d2.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+    L.st_filter.argtypes = [vp, _dp, _dp, ctypes.c_int]
+    L.st_filter_dir.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.st_gauss.argtypes = [vp, _dp, _dp, ctypes.c_int]
+    L.st_gauss_dir.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.st_rhs.argtypes = [vp, _dp, _dp]
+    L.st_hyper_rhs.argtypes = [vp, _dp]
+    L.st_get_hyper.argtypes = [vp, _dp]
+    L.st_set_hyper.argtypes = [vp, _dp]
+    L.st_get_grad.argtypes = [vp, _dp]
+    L.st_fix_pressure.argtypes = [vp, _dp]
+    L.st_cfl.argtypes = [vp, _dp, _dp]
+    L.st_tstep.argtypes = [vp, ctypes.c_double, ctypes.c_int, _dp]
+    L.st_rk_stage.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.st_step.argtypes = [vp, ctypes.c_double, _dp, ip]
+    L.st_march.argtypes = [vp, ctypes.c_int, _dp, ip, _dp]
+    L.st_diagnostics.argtypes = [vp, _dp]
+    L.st_profile.argtypes = [vp, ctypes.c_int]
+    L.st_profile_read.argtypes = [vp, ctypes.c_int, ip, _dp, _dp]
+    L.st_pass_name.argtypes = [ctypes.c_int]
+    L.st_pass_name.restype = ctypes.c_char_p
+    L.st_ic_isotropic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, _dp]
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim == 1):
+        raise StError("arrays must be float64, Fortran-ordered")
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def ic_isotropic(nx, ny, nz, k0=4.0, urms=1.0, mt=0.3, gamma=1.4, seed=12345, period=(2 * np.pi,) * 3):
+    """Seeded decaying-turbulence state Q(My,Mz,Mx,5) (host, deterministic per seed)."""
+    Q = np.zeros((ny, nz, nx, 5), dtype=np.float64, order="F")
+    per = (ctypes.c_double * 3)(*period)
+    r = lib().st_ic_isotropic(nx, ny, nz, ctypes.cast(per, _dp), k0, urms, mt, gamma, seed, _p(Q))
+    if r:
+        raise StError("st_ic_isotropic failed (%d)" % r)
+    return Q
+
+
+def re_for_taylor(re_lambda, k0, urms, mt, gamma=1.4, vis_exp=0.76):
+    """Reynolds number giving Taylor-microscale Reynolds number re_lambda at t=0.
+
+    For E(k) ~ k^4 exp(-2 (k/k0)^2) the Taylor microscale is lambda = 2/k0, and
+    the kinematic viscosity is ((gamma-1) T)^vis_exp / Re = c^(2 vis_exp) / Re
+    with c = 1/mt at rho = 1.
+    """
+    lam = 2.0 / k0
+    nu = urms * lam / re_lambda
+    return (1.0 / mt ** 2) ** vis_exp / nu
+
+
+class Solver:
+    """One device context (grid + parameters + resident state)."""
+
+    def __init__(self, nx, ny, nz, period=(2 * np.pi,) * 3, Re=100.0, Pr=0.7, gamma=1.4, vis_exp=0.76,
+                 bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5, d2_mode=0, rk_scheme=0, start=(0.0, 0.0, 0.0)):
+        self.grid = Grid(nx, ny, nz, period[0], period[1], period[2], start[0], start[1], start[2])
+        self.params = Params(Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal, d2_mode, rk_scheme)
+        self.shape = (ny, nz, nx)
+        h = ctypes.c_void_p()
+        r = lib().st_ctx_create(ctypes.byref(self.grid), ctypes.byref(self.params), ctypes.byref(h))
+        self.h = h
+        if r:
+            msg = lib().st_last_error(h).decode() if h.value else "context creation failed"
+            self.close()
+            raise StError("st_ctx_create: %s (%d)" % (msg, r))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().st_ctx_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, r, what):
+        if r < 0:
+            raise StError("%s: %s (%d)" % (what, lib().st_last_error(self.h).decode(), r))
+        return r
+
+    def _z(self, nf=None):
+        return np.zeros(self.shape if nf is None else self.shape + (nf,), dtype=np.float64, order="F")
+
+    # state
+    def upload(self, Q):
+        Q = _f(Q)
+        self._chk(lib().st_state_upload(self.h, _p(Q)), "st_state_upload")
+
+    def download(self):
+        Q = self._z(5)
+        self._chk(lib().st_state_download(self.h, _p(Q)), "st_state_download")
+        return Q
+
+    # operators: DFDX/DFDY/DFDZ, D2FDX2..., filters
+    def dfd(self, d, F, dF=None, iadd=0, cadd=1.0):
+        F = _f(F)
+        out = self._z() if dF is None else np.array(dF, dtype=np.float64, order="F", copy=True)
+        self._chk(lib().st_d1(self.h, d, _p(F), _p(out), iadd, cadd), "st_d1")
+        return out
+
+    def d2fd2(self, d, F, iadd=0):
+        F = _f(F)
+        dF, d2F = self._z(), self._z()
+        self._chk(lib().st_d2(self.h, d, _p(F), _p(dF), _p(d2F), iadd), "st_d2")
+        return (dF, d2F) if iadd else d2F
+
+    def filter_dir(self, d, F):
+        F = _f(F)
+        out = self._z()
+        self._chk(lib().st_filter_dir(self.h, d, _p(F), _p(out)), "st_filter_dir")
+        return out
+
+    def gauss_dir(self, d, F):
+        F = _f(F)
+        out = self._z()
+        self._chk(lib().st_gauss_dir(self.h, d, _p(F), _p(out)), "st_gauss_dir")
+        return out
+
+    def compact_filter(self, F):
+        F = _f(F)
+        nf = F.shape[3] if F.ndim == 4 else 1
+        out = np.zeros_like(F, order="F")
+        self._chk(lib().st_filter(self.h, _p(F), _p(out), nf), "st_filter")
+        return out
+
+    def gaussian_filter(self, F):
+        F = _f(F)
+        nf = F.shape[3] if F.ndim == 4 else 1
+        out = np.zeros_like(F, order="F")
+        self._chk(lib().st_gauss(self.h, _p(F), _p(out), nf), "st_gauss")
+        return out
+
+    # physics
+    def rhs(self, Q=None):
+        out = self._z(5)
+        Qf = None if Q is None else _f(Q)
+        self._chk(lib().st_rhs(self.h, _p(Qf), _p(out)), "st_rhs")
+        return out
+
+    def hyper_rhs(self, Qrhs):
+        out = np.array(Qrhs, dtype=np.float64, order="F", copy=True)
+        self._chk(lib().st_hyper_rhs(self.h, _p(out)), "st_hyper_rhs")
+        return out
+
+    def full_rhs(self, Q=None):
+        return self.hyper_rhs(self.rhs(Q))
+
+    def grad(self):
+        out = self._z(12)
+        self._chk(lib().st_get_grad(self.h, _p(out)), "st_get_grad")
+        return out
+
+    def hyper(self):
+        out = self._z(3)
+        self._chk(lib().st_get_hyper(self.h, _p(out)), "st_get_hyper")
+        return out
+
+    def set_hyper(self, H):
+        H = _f(H)
+        self._chk(lib().st_set_hyper(self.h, _p(H)), "st_set_hyper")
+
+    def fix_pressure(self, Q):
+        out = np.array(Q, dtype=np.float64, order="F", copy=True)
+        self._chk(lib().st_fix_pressure(self.h, _p(out)), "st_fix_pressure")
+        return out
+
+    def cfl(self):
+        a, b = np.zeros(1), np.zeros(1)
+        self._chk(lib().st_cfl(self.h, _p(a), _p(b)), "st_cfl")
+        return a[0], b[0]
+
+    def tstep(self, cfl_resv, step):
+        dt = np.zeros(1)
+        self._chk(lib().st_tstep(self.h, cfl_resv, step, _p(dt)), "st_tstep")
+        return dt[0]
+
+    def rk_stage(self, alpha, beta, dt, reset_qw=False):
+        self._chk(lib().st_rk_stage(self.h, alpha, beta, dt, 1 if reset_qw else 0), "st_rk_stage")
+
+    def lddrk4(self, dt, time=0.0, step=0):
+        t = np.array([time])
+        s = ctypes.c_int(step)
+        self._chk(lib().st_step(self.h, dt, _p(t), ctypes.byref(s)), "st_step")
+        return t[0], s.value
+
+    def march(self, nsteps, time=0.0, step=0):
+        t = np.array([time])
+        s = ctypes.c_int(step)
+        dts = np.zeros(max(nsteps, 1))
+        self._chk(lib().st_march(self.h, nsteps, _p(t), ctypes.byref(s), _p(dts)), "st_march")
+        return t[0], s.value, dts[:nsteps]
+
+    def diagnostics(self):
+        out = np.zeros(7)
+        self._chk(lib().st_diagnostics(self.h, _p(out)), "st_diagnostics")
+        return out
+
+    def sync(self):
+        self._chk(lib().st_sync(self.h), "st_sync")
+
+    # profiling
+    def profile(self, on=True):
+        self._chk(lib().st_profile(self.h, 1 if on else 0), "st_profile")
+
+    def profile_read(self):
+        n = 24
+        cnt = (ctypes.c_int * n)()
+        ms = np.zeros(n)
+        fr = np.zeros(n)
+        k = self._chk(lib().st_profile_read(self.h, n, cnt, _p(ms), _p(fr)), "st_profile_read")
+        out = {}
+        for i in range(k):
+            if cnt[i]:
+                out[lib().st_pass_name(i).decode()] = {"count": cnt[i], "ms": ms[i], "fields": fr[i]}
+        return out
