@@ -1,0 +1,156 @@
+"""Benchmark of the per-RK-stage pipeline on one MI355X.
+
+Workload: 256^3 triply periodic decaying isotropic turbulence, FP64
+(k0 = 8, M_t = 0.5, u_rms = 1, Re_lambda ~ 40 at t = 0, Pr = 0.7,
+vis_exp = 0.76, artificial transport on, CFL 0.5).  A benchmark step is one
+march-mode-1 step of the reference driver: Compute_CFL, Compute_Tstep, then
+LDDRK4 = 5 + 6 RK stages (RHS, Hyper_RHS, update, compact filter,
+pressure fix).  The metric is grid points advanced through one RK stage
+per second (Mpts*stage/s), with the state resident in HBM.
+
+Prints one JSON line.  With --gpus N > 1 it exits: this build measures a
+single GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "shock-turbulence_amd", "python"))
+
+B_ALG = 2136.0          # algorithmic bytes per grid point per RK stage (FP64)
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+STAGES_PER_STEP = 11
+
+
+def cpu_baseline(n, threads, Q, kw):
+    """Time the CPU oracle (C restatement, OpenMP) on one RK stage of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    O.lib().or_threads(threads)
+    o = O.Oracle(n, n, n, **kw)
+    stab, resv = o.cfl(Q)
+    dt = o.tstep(Q, resv, 0)
+    QW = np.zeros_like(Q)
+    t0 = time.perf_counter()
+    o.rk_stage(Q, QW, 0.0, 0.1028639988104959, dt)
+    el = time.perf_counter() - t0
+    pts = float(n) ** 3
+    return {"value": round(pts / el / 1e6, 3), "unit": "Mpts*stage/s", "cores": threads, "kind": "port",
+            "sample": "one full RK stage (RHS + Hyper_RHS + update + filter + Fix_Pressure) at %d^3 of the "
+                      "same initial state, oracle/st_oracle.c with %d OpenMP threads, %.2f s" % (n, threads, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--profile", type=int, default=1, help="per-pass HIP-event timing in the timed region")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 2 GiB device-to-device copy first (reference kernel for rocprofv3 byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; --gpus %d is not supported in this build" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import st_amd
+
+    if args.calib_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+
+    n = args.n
+    k0, mt, urms = 8.0, 0.5, 1.0
+    re = st_amd.re_for_taylor(40.0, k0, urms, mt)
+    kw = dict(Re=re, Pr=0.7, gamma=1.4, vis_exp=0.76, bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5)
+    t0 = time.perf_counter()
+    Q = st_amd.ic_isotropic(n, n, n, k0=k0, urms=urms, mt=mt, seed=12345)
+    t_ic = time.perf_counter() - t0
+    s = st_amd.Solver(n, n, n, **kw)
+    s.upload(Q)
+    time_, step = 0.0, 0
+    if args.warmup:
+        time_, step, _ = s.march(args.warmup, time_, step)
+    s.sync()
+    if args.profile:
+        s.profile(True)
+    s.sync()
+    t0 = time.perf_counter()
+    time_, step, dts = s.march(args.steps, time_, step)
+    s.sync()
+    el = time.perf_counter() - t0
+    prof = s.profile_read() if args.profile else {}
+    diag = s.diagnostics()
+    Qend = s.download()
+    finite = bool(np.isfinite(Qend).all())
+    s.close()
+
+    pts = float(n) ** 3
+    stages = args.steps * STAGES_PER_STEP
+    value = pts * stages / el / 1e6
+    ms_step = el / args.steps * 1e3
+
+    # dominant kernel: largest share of the timed region
+    roof = None
+    kernels = {}
+    if prof:
+        for name, p in prof.items():
+            avg = p["ms"] / p["count"]
+            alg = p["fields"] * 8.0 * pts
+            kernels[name] = {"launches": p["count"], "total_ms": round(p["ms"], 3), "avg_ms": round(avg, 4),
+                             "alg_bytes_per_launch": alg,
+                             "achieved_GBs": round(alg / (avg * 1e-3) / 1e9, 1) if alg else None}
+        dom = max((k for k in kernels if kernels[k]["alg_bytes_per_launch"]), key=lambda k: kernels[k]["total_ms"])
+        d = kernels[dom]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
+    stage_roof = {"B_alg_per_pt_stage": B_ALG, "achieved_GBs": round(B_ALG * pts * stages / el / 1e9, 1),
+                  "frac": round(B_ALG * pts * stages / el / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n, args.cpu_threads, Q, kw)
+
+    out = {
+        "metric": "Mpts*RK-stage/s FP64 (256^3 periodic RHS+LDDRK4 stage throughput)",
+        "value": round(value, 2),
+        "unit": "Mpts*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d^3 periodic decaying isotropic turbulence, k0=8, Mt=0.5, u_rms=1, "
+                               "Re_lambda=40 at t=0 (Re=%.1f), Pr=0.7, hyper on, CFL 0.5; "
+                               "step = CFL + dt + LDDRK4 (11 stages)" % (n, re),
+                   "grid": [n, n, n], "stages_per_step": STAGES_PER_STEP, "ic_seconds": round(t_ic, 2)},
+        "roofline": roof,
+        "roofline_stage": stage_roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "check": {"finite": finite, "sim_time": time_, "step": step, "dt_last": float(dts[-1]) if len(dts) else None,
+                  "diagnostics": [float(x) for x in diag]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

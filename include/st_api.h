@@ -103,8 +103,9 @@ int st_march(st_ctx *ctx, int nsteps, double *time, int *step, double *dt_out);
 int st_diagnostics(st_ctx *ctx, double *out7);
 
 /* Per-pass GPU timing with HIP events (0 off, 1 on).  st_profile_read
- * returns, for up to nmax pass kinds, the launch count, total milliseconds
- * and the fields read+written per launch; names via st_pass_name. */
+ * returns, for up to nmax entries (pass kind x direction), the launch count,
+ * total milliseconds and the fields read+written per launch; names
+ * ("kind.direction") via st_pass_name. */
 int st_profile(st_ctx *ctx, int on);
 int st_profile_read(st_ctx *ctx, int nmax, int *count, double *total_ms, double *fields_rw);
 const char *st_pass_name(int kind);

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -26,7 +27,7 @@ const char *kind_names[NKIND] = {
 
 struct Event {
     hipEvent_t a, b;
-    int kind;
+    int kind;             // pass kind * 3 + direction
 };
 
 }  // namespace
@@ -39,7 +40,8 @@ struct st_ctx {
     double sx1[3];
     st::Phys ph;
     st::Coef C;
-    st::DirTab th[6];               // [dir] 256-thread layout, [3+dir] 512-thread layout
+    st::DirTab th[6];               // [dir] M points per lane, [3+dir] M/2 points per lane
+    int cfg_narrow[NKIND], cfg_nt[NKIND];
     st::DirTab *td = nullptr;
     double *buf = nullptr;
     double *Q, *QW, *dQ, *G, *L2e, *S, *Cb, *L2S, *L4e, *L4S, *eta, *Hc, *T5, *tB, *tC;
@@ -50,9 +52,9 @@ struct st_ctx {
     int prof = 0;
     std::vector<Event> ev;
     std::vector<hipEvent_t> pool;
-    double prof_ms[NKIND];
-    long prof_cnt[NKIND];
-    double prof_fields[NKIND];
+    double prof_ms[NKIND * 3];      // per pass kind and direction
+    long prof_cnt[NKIND * 3];
+    double prof_fields[NKIND * 3];
 };
 
 namespace {
@@ -166,17 +168,18 @@ double kind_fields(int kind, const st::PassArgs &a)
 
 int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
 {
-    const int lay = st::pass_narrow(kind) ? 3 : 0;
+    const int pk = kind * 3 + dir;
+    const int lay = c->cfg_narrow[kind] ? 3 : 0;
     const st::DirTab &t = c->th[lay + dir];
     a.tab = c->td + lay + dir;
     a.K1 = cadd * c->sx1[dir] * (double)(t.n - 1) * t.op[0].scale;   // fac of src/derivCDopt6_periodic.f:181
     a.K2 = t.s2 * t.op[1].scale;
     Event e;
-    if (c->prof) prof_begin(c, kind, e);
-    hipError_t err = st::run_pass(kind, t, dir, c->My, c->Mz, c->Mx, a, c->s);
+    if (c->prof) prof_begin(c, pk, e);
+    hipError_t err = st::run_pass(kind, t, c->cfg_nt[kind], dir, c->My, c->Mz, c->Mx, a, c->s);
     if (c->prof) {
         prof_end(c, e);
-        c->prof_fields[kind] = kind_fields(kind, a);
+        c->prof_fields[pk] = kind_fields(kind, a);
     }
     if (err != hipSuccess) return fail(c, -4, "pass %s dir %d: %s", kind_names[kind], dir, hipGetErrorString(err));
     return 0;
@@ -205,6 +208,15 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
     std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
     std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
+    // launch layouts; ST_NARROW / ST_NT512 (bit masks over pass kinds) override
+    {
+        const char *en = getenv("ST_NARROW"), *et = getenv("ST_NT512");
+        for (int k = 0; k < NKIND; k++) {
+            st::pass_default_cfg(k, &c->cfg_narrow[k], &c->cfg_nt[k]);
+            if (en) c->cfg_narrow[k] = (int)((strtoul(en, nullptr, 0) >> k) & 1);
+            if (et) c->cfg_nt[k] = ((strtoul(et, nullptr, 0) >> k) & 1) ? 512 : 256;
+        }
+    }
     const int n[3] = {c->Mx, c->My, c->Mz};
     const double period[3] = {grid->period_x, grid->period_y, grid->period_z};
     const double start[3] = {grid->x_start, grid->y_start, grid->z_start};
@@ -276,9 +288,9 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
         const double sx1 = 1.0 / leng_of(period[d], n[d]);   // src/grid.f:31-32
         const double sx2 = sx1 * sx1;
         c->sx1[d] = sx1;
-        if (st::build_dirtab(&c->th[d], n[d], M, P, 256 / P, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0, 6.6624e-01, 1.6688e-01) ||
+        if (st::build_dirtab(&c->th[d], n[d], M, P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0, 6.6624e-01, 1.6688e-01) ||
             (M % 2 == 0 && 2 * P <= 64 &&
-             st::build_dirtab(&c->th[3 + d], n[d], M / 2, 2 * P, 512 / (2 * P), sx1, sx2, 1.0 / 3.0, 2.0 / 11.0,
+             st::build_dirtab(&c->th[3 + d], n[d], M / 2, 2 * P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0,
                               6.6624e-01, 1.6688e-01))) {
             fail(c, -2, "coefficient tables failed for direction %d", d);
             *out = c;
@@ -405,7 +417,7 @@ int down(st_ctx *c, double *host, const double *dev, long nfields)
 int fill(st_ctx *c, double *p, long n, double v)
 {
     Event e;
-    if (c->prof) prof_begin(c, KIND_FILL, e);
+    if (c->prof) prof_begin(c, KIND_FILL * 3, e);
     hipError_t err = st::run_fill(p, n, v, c->s);
     if (c->prof) prof_end(c, e);
     if (err != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(err));
@@ -431,11 +443,11 @@ int seq_rhs(st_ctx *c)
     TRY(pass(c, st::K_GRAD, 0, a));
     {
         Event e;
-        if (c->prof) prof_begin(c, KIND_CROSS, e);
+        if (c->prof) prof_begin(c, KIND_CROSS * 3, e);
         hipError_t err = st::run_cross(a, hyp, c->s);
         if (c->prof) {
             prof_end(c, e);
-            c->prof_fields[KIND_CROSS] = kind_fields(KIND_CROSS, a);
+            c->prof_fields[KIND_CROSS * 3] = kind_fields(KIND_CROSS, a);
         }
         if (err != hipSuccess) return fail(c, -4, "cross: %s", hipGetErrorString(err));
     }
@@ -492,7 +504,7 @@ int reduce(st_ctx *c, int which, int nv, double *out)
     st::PassArgs a = base_args(c);
     int nb = 0;
     Event e;
-    if (c->prof) prof_begin(c, KIND_RED, e);
+    if (c->prof) prof_begin(c, KIND_RED * 3, e);
     hipError_t err = st::run_reduce(which, a, c->red, &nb, c->s);
     if (c->prof) prof_end(c, e);
     if (err != hipSuccess) return fail(c, -4, "reduce: %s", hipGetErrorString(err));
@@ -772,7 +784,7 @@ extern "C" int st_profile_read(st_ctx *c, int nmax, int *count, double *total_ms
     if (!c) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     prof_collect(c);
-    const int n = nmax < NKIND ? nmax : NKIND;
+    const int n = nmax < NKIND * 3 ? nmax : NKIND * 3;
     for (int i = 0; i < n; i++) {
         if (count) count[i] = (int)c->prof_cnt[i];
         if (total_ms) total_ms[i] = c->prof_ms[i];
@@ -783,8 +795,10 @@ extern "C" int st_profile_read(st_ctx *c, int nmax, int *count, double *total_ms
 
 extern "C" const char *st_pass_name(int kind)
 {
-    if (kind < 0 || kind >= NKIND) return "";
-    return kind_names[kind];
+    static char buf[NKIND * 3][32];
+    if (kind < 0 || kind >= NKIND * 3) return "";
+    if (!buf[kind][0]) snprintf(buf[kind], sizeof(buf[kind]), "%s.%c", kind_names[kind / 3], "xyz"[kind % 3]);
+    return buf[kind];
 }
 
 extern "C" int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,

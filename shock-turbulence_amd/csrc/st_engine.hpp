@@ -52,19 +52,21 @@ DEV long tile_of(const Geo &g)
     return wg;
 }
 
-// LDS layout of one slot: line b, point s = c*M + k stored at
-// b*(M*(P+1)) + k*(P+1) + c, so lane-parallel reads at fixed k are
-// consecutive addresses.
+// LDS layout of one slot: line b holds its n = M*P points in a block of
+// n doubles; point s = c*M + k sits at row k, column c.  Rows are rotated by
+// b and columns XOR-swizzled so that lane-parallel reads at fixed k and the
+// memory-order staging writes both spread over the 32 double-wide banks.
 template <int M>
 HD int lds_idx(int b, int k, int c, int P)
 {
-    return b * (M * (P + 1)) + k * (P + 1) + c;
+    const int f = k * (P / M) + (M == 4 ? 4 * b : b);
+    return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
 }
 
 template <int M>
 HD int slot_size(int P, int B)
 {
-    return B * M * (P + 1);
+    return B * M * P;
 }
 
 // Element e (0 .. B*n-1) of the tile in memory order -> (b, s).

@@ -38,9 +38,10 @@ enum PassKind {
     K_RKF_X, K_F_Y, K_F_Z,
 };
 
-// true for passes launched with 512 threads (tables built for M/2, 2P)
-bool pass_narrow(int kind);
-hipError_t run_pass(int kind, const DirTab &t, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+// Default launch layout of a pass kind: narrow = 1 uses the tables with half
+// the points per lane (twice the lanes per line); nt = threads per workgroup.
+void pass_default_cfg(int kind, int *narrow, int *nt);
+hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s);
 
 }  // namespace st

@@ -18,13 +18,19 @@ namespace st {
 // generic pass kernel
 // ---------------------------------------------------------------------
 template <class PASS>
-constexpr int pass_slots() { return PASS::NSLOT > PASS::R ? PASS::NSLOT : PASS::R; }
+constexpr int pass_slots() { return PASS::NSLOT + PASS::NEARLY; }
 
+// Slots 0..NSLOT-1 hold the staged inputs; compute() may write final
+// results early into slots NSLOT..NSLOT+NEARLY-1 (freeing their registers);
+// the NOUT register results are staged into slots 0..NOUT-1 after compute.
+// store() receives, per point, the NOUT late results followed by the
+// NEARLY early ones.
 template <int M, int NT, class PASS>
 __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
 {
     extern __shared__ double lds[];
-    constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, R = PASS::R;
+    constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
+    static_assert(NO <= NS, "late outputs must fit in the input slots");
     const long tile = tile_of(g);
     const long line0 = tile * g.B;
     const int ss = slot_size<M>(g.P, g.B);
@@ -39,8 +45,9 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
         double v[NS > 0 ? NS : 1];
         PASS::load(g, a, gi, v);
         const int k = s % M, c = s / M;
+        const int li = lds_idx<M>(b, k, c, g.P);
 #pragma unroll
-        for (int q = 0; q < NS; q++) lds[q * ss + lds_idx<M>(b, k, c, g.P)] = v[q];
+        for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
     }
     __syncthreads();
 
@@ -50,28 +57,27 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
     PASS::template compute<M>(g, a, L, out);
 
     // 3. store (through LDS, memory order)
+    __syncthreads();
 #pragma unroll
-    for (int j0 = 0; j0 < NO; j0 += R) {
-        __syncthreads();
+    for (int j = 0; j < NO; j++) {
 #pragma unroll
-        for (int j = 0; j < R; j++)
-            if (j0 + j < NO) {
-#pragma unroll
-                for (int k = 0; k < M; k++) L.st(j, k, out[j0 + j][k]);
-            }
-        __syncthreads();
+        for (int k = 0; k < M; k++) L.st(j, k, out[j][k]);
+    }
+    __syncthreads();
 #pragma unroll 2
-        for (int r = 0; r < M; r++) {
-            const int e = threadIdx.x + NT * r;
-            int b, s;
-            tile_elem(g, e, b, s);
-            const long gi = gindex(g, line0 + b, s);
-            const int k = s % M, c = s / M;
-            double v[R];
+    for (int r = 0; r < M; r++) {
+        const int e = threadIdx.x + NT * r;
+        int b, s;
+        tile_elem(g, e, b, s);
+        const long gi = gindex(g, line0 + b, s);
+        const int k = s % M, c = s / M;
+        const int li = lds_idx<M>(b, k, c, g.P);
+        double v[NO + NE];
 #pragma unroll
-            for (int j = 0; j < R; j++) v[j] = (j0 + j < NO) ? lds[j * ss + lds_idx<M>(b, k, c, g.P)] : 0.0;
-            PASS::store(g, a, gi, j0, v);
-        }
+        for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
+#pragma unroll
+        for (int j = 0; j < NE; j++) v[NO + j] = lds[(NS + j) * ss + li];
+        PASS::store(g, a, gi, v);
     }
 }
 
@@ -100,7 +106,7 @@ DEV void upd(double *p, double v, bool first)
 // entry points of the C-ABI)
 // ---------------------------------------------------------------------
 struct PassD1 {
-    static constexpr int NSLOT = 1, NOUT = 1, R = 1;
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
     DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
@@ -111,30 +117,32 @@ struct PassD1 {
         window<M>(f, L, w);
         op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, out[0]);
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         a.out0[gi] = a.iadd ? a.out0[gi] + v[0] : v[0];
     }
 };
 
 struct PassD2 {
-    static constexpr int NSLOT = 1, NOUT = 2, R = 2;
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 1;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[2][M])
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
     {
-        double f[M], w[M + 2 * HALO];
+        double f[M], w[M + 2 * HALO], d2[M];
 #pragma unroll
         for (int k = 0; k < M; k++) f[k] = L.ld(0, k);
         window<M>(f, L, w);
-        op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[1]);
+        op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, d2);
+#pragma unroll
+        for (int k = 0; k < M; k++) L.st(1, k, d2[k]);
         if (a.iadd) op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, out[0]);
         else {
 #pragma unroll
             for (int k = 0; k < M; k++) out[0][k] = 0.0;
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         if (a.iadd) a.out0[gi] = v[0];
         a.out1[gi] = v[1];
@@ -142,7 +150,7 @@ struct PassD2 {
 };
 
 struct PassFilt {
-    static constexpr int NSLOT = 1, NOUT = 1, R = 1;
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
     DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
@@ -153,11 +161,11 @@ struct PassFilt {
         window<M>(f, L, w);
         op_filt<M>(w, a.C, a.tab->op[2].scale, a.tab->op[2], L, out[0]);
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v) { a.out0[gi] = v[0]; }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v) { a.out0[gi] = v[0]; }
 };
 
 struct PassGauss {
-    static constexpr int NSLOT = 1, NOUT = 1, R = 1;
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
     DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
@@ -168,7 +176,7 @@ struct PassGauss {
         window<M>(f, L, w);
         op_gauss<M>(w, a.C, out[0]);
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v) { a.out0[gi] = v[0]; }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v) { a.out0[gi] = v[0]; }
 };
 
 // ---------------------------------------------------------------------
@@ -189,7 +197,7 @@ struct PassGauss {
 // evaluation only by rounding.
 // ---------------------------------------------------------------------
 struct PassGrad {
-    static constexpr int NSLOT = 5, NOUT = 10, R = 5;
+    static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
@@ -199,61 +207,15 @@ struct PassGrad {
         v[3] = pr.T;
         v[4] = a.Q[3 * a.N + gi];
     }
-    // u, v, w (COMP 0..2) or T (COMP 3): first and second derivative
-    template <int COMP, int M>
-    DEV static void velT(const Geo &g, const PassArgs &a, const Lane<M> &L, const double (&mu)[M], double (&out)[10][M])
+    // Accumulate into row i of out with a static register index.
+    template <int M>
+    DEV static void acc(double (&out)[5][M], int i, int k, double v)
     {
-        const Phys &ph = a.ph;
-        const int d = g.dir;
-        double f[M], w[M + 2 * HALO], gd[M], hd[M];
 #pragma unroll
-        for (int k = 0; k < M; k++) f[k] = L.ld(COMP, k);
-        window<M>(f, L, w);
-        op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
-        op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
-#pragma unroll
-        for (int k = 0; k < M; k++) {
-            out[5 + COMP][k] = gd[k];
-            if (COMP < 3) {
-                const double rho = L.ld(4, k), ud = L.ld(d, k);
-                const double wd = COMP == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
-                const double vt = wd * mu[k] * hd[k];
-                out[COMP][k] += vt + (-0.5 * rho * ud) * gd[k];
-                out[4][k] += f[k] * vt;
-            } else {
-                const double mup = dvisc(ph, f[k]);
-                out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
-            }
-        }
-    }
-    // flux derivative: COMP 0..2 momentum (with pressure), 4 energy
-    template <int COMP, int M>
-    DEV static void flux(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[10][M])
-    {
-        const Phys &ph = a.ph;
-        const int d = g.dir;
-        double f[M], w[M + 2 * HALO], df[M];
-#pragma unroll
-        for (int k = 0; k < M; k++) {
-            const double u = L.ld(0, k), v = L.ld(1, k), ww = L.ld(2, k), T = L.ld(3, k), rho = L.ld(4, k);
-            const double ud = L.ld(d, k);
-            double fl;
-            if (COMP < 3) {
-                const double ui = COMP == 0 ? u : COMP == 1 ? v : ww;
-                fl = -0.5 * rho * ui * ud;
-                if (COMP == d) fl -= rho * T * ph.gm1 * ph.ginv;
-            } else {
-                fl = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
-            }
-            f[k] = fl;
-        }
-        window<M>(f, L, w);
-        op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, df);
-#pragma unroll
-        for (int k = 0; k < M; k++) out[COMP][k] += df[k];
+        for (int r = 0; r < 5; r++) out[r][k] += (i == r) ? v : 0.0;
     }
     template <int M>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[10][M])
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -264,17 +226,43 @@ struct PassGrad {
 #pragma unroll
             for (int i = 0; i < 5; i++) out[i][k] = 0.0;
         }
-        velT<0, M>(g, a, L, mu, out);
-        velT<1, M>(g, a, L, mu, out);
-        velT<2, M>(g, a, L, mu, out);
-        velT<3, M>(g, a, L, mu, out);
+        // u, v, w, T: first and second derivatives (one operator pair at a time)
+#pragma unroll 1
+        for (int comp = 0; comp < 4; comp++) {
+            double f[M], w[M + 2 * HALO], gd[M], hd[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
+            if (comp < 3) {
+                const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    L.st(5 + comp, k, gd[k]);
+                    const double rho = L.ld(4, k), ud = L.ld(d, k);
+                    const double vt = wd * mu[k] * hd[k];
+                    acc<M>(out, comp, k, vt + (-0.5 * rho * ud) * gd[k]);
+                    out[4][k] += f[k] * vt;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    L.st(8, k, gd[k]);
+                    const double mup = dvisc(ph, f[k]);
+                    out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
+                }
+            }
+        }
         // internal energy e = T/gamma: Laplacian contribution
         {
-            double f[M], w[M + 2 * HALO];
+            double f[M], w[M + 2 * HALO], le[M];
 #pragma unroll
             for (int k = 0; k < M; k++) f[k] = L.ld(3, k) * ph.ginv;
             window<M>(f, L, w);
-            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[9]);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, le);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(9, k, le[k]);
         }
         // skew-symmetric part and continuity with the mass flux rho u_d
         {
@@ -292,22 +280,39 @@ struct PassGrad {
                 out[3][k] = -sk[k];
             }
         }
-        flux<0, M>(g, a, L, out);
-        flux<1, M>(g, a, L, out);
-        flux<2, M>(g, a, L, out);
-        flux<4, M>(g, a, L, out);
-    }
-    DEV static void store(const Geo &g, const PassArgs &a, long gi, int j0, const double *v)
-    {
-        if (j0 == 0) {
+        // flux derivatives: momentum (with pressure) 0..2, energy 4
+#pragma unroll 1
+        for (int comp = 0; comp < 5; comp++) {
+            if (comp == 3) continue;
+            double f[M], w[M + 2 * HALO], df[M];
 #pragma unroll
-            for (int c = 0; c < 5; c++) upd(&a.dQ[c * a.N + gi], v[c], a.first);
-        } else {
+            for (int k = 0; k < M; k++) {
+                const double u = L.ld(0, k), v = L.ld(1, k), ww = L.ld(2, k), T = L.ld(3, k), rho = L.ld(4, k);
+                const double ud = L.ld(d, k);
+                double fl;
+                if (comp < 3) {
+                    const double ui = L.ld(comp, k);
+                    fl = -0.5 * rho * ui * ud;
+                    if (comp == d) fl -= rho * T * ph.gm1 * ph.ginv;
+                } else {
+                    fl = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
+                }
+                f[k] = fl;
+            }
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, df);
 #pragma unroll
-            for (int c = 0; c < 3; c++) a.G[(g.dir * 3 + c) * a.N + gi] = v[c];
-            a.G[(9 + g.dir) * a.N + gi] = v[3];
-            if (a.L2e) upd(&a.L2e[gi], v[4], a.first);
+            for (int k = 0; k < M; k++) acc<M>(out, comp, k, df[k]);
         }
+    }
+    DEV static void store(const Geo &g, const PassArgs &a, long gi, const double *v)
+    {
+#pragma unroll
+        for (int c = 0; c < 5; c++) upd(&a.dQ[c * a.N + gi], v[c], a.first);
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.G[(g.dir * 3 + c) * a.N + gi] = v[5 + c];
+        a.G[(9 + g.dir) * a.N + gi] = v[8];
+        if (a.L2e) upd(&a.L2e[gi], v[9], a.first);
     }
 };
 
@@ -323,7 +328,7 @@ template <int DIR, bool HYP>
 struct PassMix {
     static constexpr int NMIX = DIR == 1 ? 4 : DIR == 2 ? 2 : 0;
     static constexpr int NH = HYP ? 2 : 0;
-    static constexpr int NSLOT = NMIX + NH, NOUT = NMIX + NH, R = NOUT;
+    static constexpr int NSLOT = NMIX + NH, NOUT = NMIX + NH, NEARLY = 0;
     // G index of each mixed input: gxx=0, gyx=1, gzx=2, gxy=3, gyy=4, gzy=5, gxz=6, gyz=7, gzz=8
     DEV static int gsrc(int m)
     {
@@ -352,7 +357,7 @@ struct PassMix {
             else op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[m]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         if (NMIX > 0) {
             const Phys &ph = a.ph;
@@ -387,7 +392,7 @@ struct PassMix {
 // ---------------------------------------------------------------------
 template <bool LAST>
 struct PassLap2 {
-    static constexpr int NSLOT = 1, NOUT = 1, R = 1;
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
     template <int M>
     DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
@@ -398,7 +403,7 @@ struct PassLap2 {
         window<M>(f, L, w);
         op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[0]);
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         if (!LAST) {
             upd(&a.L4S[gi], v[0], a.first);
@@ -421,7 +426,7 @@ struct PassLap2 {
 // ---------------------------------------------------------------------
 template <bool LAST>
 struct PassGaussEta {
-    static constexpr int NSLOT = 2, NOUT = 2, R = 2;
+    static constexpr int NSLOT = 2, NOUT = 2, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[2])
     {
         v[0] = a.eta[gi];
@@ -439,7 +444,7 @@ struct PassGaussEta {
             op_gauss<M>(w, a.C, out[m]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         if (!LAST) {
             a.eta[gi] = v[0];
@@ -462,7 +467,7 @@ struct PassGaussEta {
 // sum).
 // ---------------------------------------------------------------------
 struct PassHyperDiv {
-    static constexpr int NSLOT = 4, NOUT = 4, R = 4;
+    static constexpr int NSLOT = 4, NOUT = 4, NEARLY = 0;
     DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
     {
         const long N = a.N;
@@ -505,7 +510,7 @@ struct PassHyperDiv {
             op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, out[m]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         a.dQ[gi] += v[0];
         a.dQ[a.N + gi] += v[1];
@@ -521,7 +526,7 @@ struct PassHyperDiv {
 // ---------------------------------------------------------------------
 template <int KIND>   // 0: RK update + x filter, 1: y filter, 2: z filter + pressure fix
 struct PassFilterQ {
-    static constexpr int NSLOT = 5, NOUT = 5, R = 5;
+    static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
 #pragma unroll
@@ -549,7 +554,7 @@ struct PassFilterQ {
             op_filt<M>(w, a.C, T.scale, T, L, out[m]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, int, const double *v)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         const long N = a.N;
         if (KIND == 2 && a.fix) {
@@ -645,10 +650,10 @@ static hipError_t launch_m(int M, const Geo &g, const PassArgs &a, hipStream_t s
     }
 }
 
-// Wide layout: NT = 256 with the direction's (M, P); narrow layout for the
-// register-heavy passes: NT = 512, half the points per lane.
-template <class PASS, bool NARROW>
-static hipError_t launch(const DirTab &t, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+// t holds the tables of the chosen layout (M points per lane, P lanes per
+// line); nt threads per workgroup give B = nt/P lines per tile.
+template <class PASS>
+static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     Geo g;
     g.dir = dir;
@@ -657,45 +662,57 @@ static hipError_t launch(const DirTab &t, int dir, int My, int Mz, int Mx, const
     g.Mz = Mz;
     g.Mx = Mx;
     g.npts = (long)My * Mz * Mx;
-    // t holds the tables of this layout (the 512-thread layout has half the
-    // points per lane and twice the lanes per line)
-    const int M = t.M, P = t.P;
-    constexpr int NT = NARROW ? 512 : 256;
-    if (t.B != NT / P) return hipErrorInvalidValue;
-    g.P = P;
-    g.B = NT / P;
+    g.P = t.P;
+    g.B = nt / t.P;
+    if (g.B < 1 || g.B * t.P != nt) return hipErrorInvalidValue;
     const long lot = g.npts / g.n;
     if (lot % g.B) return hipErrorInvalidValue;
     g.ntiles = lot / g.B;
-    return launch_m<PASS, NT>(M, g, a, s);
+    if (nt == 256) return launch_m<PASS, 256>(t.M, g, a, s);
+    if (nt == 512) return launch_m<PASS, 512>(t.M, g, a, s);
+    return hipErrorInvalidValue;
 }
 
-bool pass_narrow(int kind)
+void pass_default_cfg(int kind, int *narrow, int *nt)
 {
-    return kind == K_GRAD || kind == K_RKF_X || kind == K_F_Y || kind == K_F_Z;
+    *nt = 256;
+    switch (kind) {
+    case K_RKF_X: case K_MIXH_Z: case K_HDIV:
+        // measured at 256^3: these lose less to the tail of the grid
+        // with 512-thread tiles than they gain from a second resident block
+        *nt = 512;
+        *narrow = 1;
+        break;
+    case K_GRAD: case K_F_Y: case K_F_Z:
+    case K_MIX_Y: case K_MIX_Z: case K_MIXH_Y:
+        *narrow = 1;
+        break;
+    default:
+        *narrow = 0;
+    }
 }
 
-hipError_t run_pass(int kind, const DirTab &t, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     switch (kind) {
-    case K_D1: return launch<PassD1, false>(t, dir, My, Mz, Mx, a, s);
-    case K_D2: return launch<PassD2, false>(t, dir, My, Mz, Mx, a, s);
-    case K_FILT: return launch<PassFilt, false>(t, dir, My, Mz, Mx, a, s);
-    case K_GAUSS: return launch<PassGauss, false>(t, dir, My, Mz, Mx, a, s);
-    case K_GRAD: return launch<PassGrad, true>(t, dir, My, Mz, Mx, a, s);
-    case K_MIX_Y: return launch<PassMix<1, false>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_MIX_Z: return launch<PassMix<2, false>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_MIXH_X: return launch<PassMix<0, true>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_MIXH_Y: return launch<PassMix<1, true>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_MIXH_Z: return launch<PassMix<2, true>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_LAP2: return launch<PassLap2<false>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_LAP2_LAST: return launch<PassLap2<true>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_GAUSS_ETA: return launch<PassGaussEta<false>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>, false>(t, dir, My, Mz, Mx, a, s);
-    case K_HDIV: return launch<PassHyperDiv, false>(t, dir, My, Mz, Mx, a, s);
-    case K_RKF_X: return launch<PassFilterQ<0>, true>(t, dir, My, Mz, Mx, a, s);
-    case K_F_Y: return launch<PassFilterQ<1>, true>(t, dir, My, Mz, Mx, a, s);
-    case K_F_Z: return launch<PassFilterQ<2>, true>(t, dir, My, Mz, Mx, a, s);
+    case K_D1: return launch<PassD1>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_D2: return launch<PassD2>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_FILT: return launch<PassFilt>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS: return launch<PassGauss>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GRAD: return launch<PassGrad>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIX_Y: return launch<PassMix<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIX_Z: return launch<PassMix<2, false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIXH_X: return launch<PassMix<0, true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIXH_Y: return launch<PassMix<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIXH_Z: return launch<PassMix<2, true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_LAP2: return launch<PassLap2<false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_LAP2_LAST: return launch<PassLap2<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA: return launch<PassGaussEta<false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_HDIV: return launch<PassHyperDiv>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_RKF_X: return launch<PassFilterQ<0>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_F_Y: return launch<PassFilterQ<1>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_F_Z: return launch<PassFilterQ<2>>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }

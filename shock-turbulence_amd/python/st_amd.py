@@ -285,7 +285,7 @@ class Solver:
         self._chk(lib().st_profile(self.h, 1 if on else 0), "st_profile")
 
     def profile_read(self):
-        n = 24
+        n = 72
         cnt = (ctypes.c_int * n)()
         ms = np.zeros(n)
         fr = np.zeros(n)
