@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "st_internal.h"
+#include "st_math.h"
 
 #define DEV __device__ __forceinline__
 #define HD __host__ __device__ __forceinline__
@@ -129,112 +130,212 @@ DEV void window(const double (&v)[M], const Lane<M> &L, double (&w)[M + 2 * HALO
 }
 
 // ---- cyclic solves ------------------------------------------------------
+//
+// The lane-to-lane traffic of a solve runs on DPP (data-parallel
+// primitives, a VALU operand modifier) and v_readlane instead of LDS
+// permutes: a scan step costs a few VALU cycles instead of an LDS round trip.
 
-// Order 1: x = (1+rho^2) (1 - rho S^-1)^-1 (1 - rho S)^-1 r; the factor
-// (1+rho^2) is already folded into r.
-template <int M>
-DEV void solve1(double (&r)[M], const OpTab &T, const Lane<M> &L)
+constexpr int DPP_ROW_SHL = 0x100;      // + d: lane i <- lane i+d within its 16-lane row
+constexpr int DPP_ROW_SHR = 0x110;      // + d: lane i <- lane i-d within its row
+constexpr int DPP_WAVE_SHL1 = 0x130;    // lane i <- lane i+1 across the wave
+constexpr int DPP_WAVE_SHR1 = 0x138;    // lane i <- lane i-1 across the wave
+constexpr int DPP_ROW_NEWBCAST = 0x150; // + k: every lane of a row <- lane k of the row
+
+// Lanes whose source lies outside the row (or the wave) read 0.
+template <int CTRL>
+DEV double dpp(double x)
 {
-    const double rho = T.a1;
-    const int P = L.P, c = L.c;
-#pragma unroll
-    for (int k = 1; k < M; k++) r[k] = fma(rho, r[k - 1], r[k]);
-    double E = r[M - 1];
-    {
-        int j = 0;
-        for (int d = 1; d < P; d <<= 1, j++) {
-            const double t = __shfl(E, L.src(-d));
-            if (c >= d) E = fma(T.sc[j][0], t, E);
-        }
-    }
-    const double Etot = __shfl(E, L.seg + P - 1) * T.wrap[0];
-    double Ep = __shfl(E, L.src(-1));
-    if (c == 0) Ep = 0.0;
-    const double cin = fma(T.lp[c][0], Etot, Ep);
-#pragma unroll
-    for (int k = 0; k < M; k++) r[k] = fma(T.cf[k][0], cin, r[k]);
-#pragma unroll
-    for (int k = M - 2; k >= 0; k--) r[k] = fma(rho, r[k + 1], r[k]);
-    double Bv = r[0];
-    {
-        int j = 0;
-        for (int d = 1; d < P; d <<= 1, j++) {
-            const double t = __shfl(Bv, L.src(d));
-            if (c + d < P) Bv = fma(T.sc[j][0], t, Bv);
-        }
-    }
-    const double Btot = __shfl(Bv, L.seg) * T.wrap[0];
-    double Bn = __shfl(Bv, L.src(1));
-    if (c == P - 1) Bn = 0.0;
-    const double bin = fma(T.lp[P - 1 - c][0], Btot, Bn);
-#pragma unroll
-    for (int k = 0; k < M; k++) r[k] = fma(T.cf[M - 1 - k][0], bin, r[k]);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
 }
 
-DEV void mv(const double *m, double x0, double x1, double &y0, double &y1)
+template <int LANE>
+DEV double rdlane(double x)
 {
-    y0 = fma(m[0], x0, m[1] * x1);
-    y1 = fma(m[2], x0, m[3] * x1);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), LANE),
+                            __builtin_amdgcn_readlane(__double2loint(x), LANE));
 }
 
-// Order 2: x = (1/kappa) (1 - a1 S^-1 - a2 S^-2)^-1 (1 - a1 S - a2 S^2)^-1 r.
-template <int M>
-DEV void solve2(double (&r)[M], const OpTab &T, const Lane<M> &L)
+// y += m t for the carried state: a scalar (order 1, m[0]) or a 2-vector
+// with the row-major 2x2 matrix m (order 2).
+template <int K>
+DEV void madd(const double *m, const double (&t)[K], double (&y)[K])
+{
+    if (K == 1) {
+        y[0] = fma(m[0], t[0], y[0]);
+    } else {
+        const double t1 = t[K - 1];
+        y[0] = fma(m[0], t[0], fma(m[1], t1, y[0]));
+        y[K - 1] = fma(m[2], t[0], fma(m[3], t1, y[K - 1]));
+    }
+}
+
+template <int K>
+DEV void madd_to(const double *m, const double (&t)[K], const double (&x)[K], double (&y)[K])
+{
+#pragma unroll
+    for (int i = 0; i < K; i++) y[i] = x[i];
+    madd<K>(m, t, y);
+}
+
+// One in-row Kogge-Stone step at distance D: forward e_c += A^(M D) e_(c-D),
+// backward e_c += A^(M D) e_(c+D).  Rows are segment boundaries for P >= 16;
+// for P = 8 two lines share a row and the cross-line source is masked.
+template <int K, bool FWD, int D, int J>
+DEV void row_step(double (&e)[K], const OpTab &T, int c, int P)
+{
+    double t[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        t[i] = dpp<FWD ? DPP_ROW_SHR + D : DPP_ROW_SHL + D>(e[i]);
+        if (P < 16 && (FWD ? c < D : c + D >= P)) t[i] = 0.0;
+    }
+    madd<K>(T.sc[J], t, e);
+}
+
+template <int K>
+DEV void sel(bool p, const double (&a)[K], const double (&b)[K], double (&y)[K])
+{
+#pragma unroll
+    for (int i = 0; i < K; i++) y[i] = p ? a[i] : b[i];
+}
+
+// Segmented scan of the chunk-end states e over the P lanes of each line.
+// On return e is the inclusive prefix (forward) or suffix (backward) state,
+// and tot the line's total (the state at its last, resp. first, lane).
+template <int K, bool FWD>
+DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c, int P)
+{
+    row_step<K, FWD, 1, 0>(e, T, c, P);
+    row_step<K, FWD, 2, 1>(e, T, c, P);
+    row_step<K, FWD, 4, 2>(e, T, c, P);
+    if (P == 8) {
+        // two lines per row: totals at row lanes 7 / 15 (forward), 0 / 8 (backward)
+        const bool hi = (threadIdx.x & 8) != 0;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const double a = FWD ? dpp<DPP_ROW_NEWBCAST + 15>(e[i]) : dpp<DPP_ROW_NEWBCAST + 8>(e[i]);
+            const double b = FWD ? dpp<DPP_ROW_NEWBCAST + 7>(e[i]) : dpp<DPP_ROW_NEWBCAST + 0>(e[i]);
+            tot[i] = hi ? a : b;
+        }
+        return;
+    }
+    row_step<K, FWD, 8, 3>(e, T, c, P);
+    if (P == 16) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+            tot[i] = FWD ? dpp<DPP_ROW_NEWBCAST + 15>(e[i]) : dpp<DPP_ROW_NEWBCAST + 0>(e[i]);
+        return;
+    }
+    // P = 32 or 64: carry whole rows with uniform values read from one
+    // lane per row (its row-local end state), A16 = A^(16 M).
+    const double *A16 = T.sc[4];
+    const int row = (threadIdx.x >> 4) & 3;
+    double s0[K], s1[K], s2[K], s3[K], carry[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        s0[i] = FWD ? rdlane<15>(e[i]) : rdlane<0>(e[i]);
+        s1[i] = FWD ? rdlane<31>(e[i]) : rdlane<16>(e[i]);
+        s2[i] = FWD ? rdlane<47>(e[i]) : rdlane<32>(e[i]);
+        s3[i] = FWD ? rdlane<63>(e[i]) : rdlane<48>(e[i]);
+    }
+    const double zero[K] = {};
+    if (P == 32) {
+        double t0[K], t1[K];
+        if (FWD) {
+            madd_to<K>(A16, s0, s1, t0);           // total of lanes 0..31
+            madd_to<K>(A16, s2, s3, t1);           // total of lanes 32..63
+            double c0[K];
+            sel<K>(row == 1, s0, zero, c0);
+            sel<K>(row == 3, s2, c0, carry);
+        } else {
+            madd_to<K>(A16, s1, s0, t0);
+            madd_to<K>(A16, s3, s2, t1);
+            double c0[K];
+            sel<K>(row == 0, s1, zero, c0);
+            sel<K>(row == 2, s3, c0, carry);
+        }
+        sel<K>(row >= 2, t1, t0, tot);
+    } else {
+        double f1[K], f2[K], c0[K], c1[K];
+        if (FWD) {
+            madd_to<K>(A16, s0, s1, f1);           // prefix through lane 31
+            madd_to<K>(A16, f1, s2, f2);           // through lane 47
+            madd_to<K>(A16, f2, s3, tot);          // through lane 63
+            sel<K>(row == 1, s0, zero, c0);
+            sel<K>(row == 2, f1, c0, c1);
+            sel<K>(row == 3, f2, c1, carry);
+        } else {
+            madd_to<K>(A16, s3, s2, f2);           // suffix from lane 32
+            madd_to<K>(A16, f2, s1, f1);           // from lane 16
+            madd_to<K>(A16, f1, s0, tot);          // from lane 0
+            sel<K>(row == 0, f1, zero, c0);
+            sel<K>(row == 1, f2, c0, c1);
+            sel<K>(row == 2, s3, c1, carry);
+        }
+    }
+    // the carry enters lane c over (c mod 16) + 1 chunks (forward) or
+    // 16 - (c mod 16) chunks (backward)
+    madd<K>(T.lp[FWD ? (c & 15) + 1 : 16 - (c & 15)], carry, e);
+}
+
+// Cyclic solve of one line per P lanes; r holds the lane's M right-hand
+// sides (the scale 1+rho^2 or 1/kappa already folded in) and returns x.
+// Order 1: x = (1 - rho S^-1)^-1 (1 - rho S)^-1 r,
+// order 2: x = (1 - a1 S^-1 - a2 S^-2)^-1 (1 - a1 S - a2 S^2)^-1 r.
+template <int K, int M>
+DEV void solve(double (&r)[M], const OpTab &T, const Lane<M> &L)
 {
     const double a1 = T.a1, a2 = T.a2;
     const int P = L.P, c = L.c;
-    if (M > 1) r[1] = fma(a1, r[0], r[1]);
+    // forward recurrence inside the chunk (zero incoming state)
 #pragma unroll
-    for (int k = 2; k < M; k++) r[k] = fma(a1, r[k - 1], fma(a2, r[k - 2], r[k]));
-    double e0 = r[M - 1], e1 = M > 1 ? r[M > 1 ? M - 2 : 0] : 0.0;
-    {
-        int j = 0;
-        for (int d = 1; d < P; d <<= 1, j++) {
-            const double t0 = __shfl(e0, L.src(-d)), t1 = __shfl(e1, L.src(-d));
-            if (c >= d) {
-                double y0, y1;
-                mv(T.sc[j], t0, t1, y0, y1);
-                e0 += y0;
-                e1 += y1;
-            }
-        }
+    for (int k = 1; k < M; k++) {
+        r[k] = fma(a1, r[k - 1], r[k]);
+        if (K == 2 && k >= 2) r[k] = fma(a2, r[k - 2], r[k]);
     }
-    double w0 = __shfl(e0, L.seg + P - 1), w1 = __shfl(e1, L.seg + P - 1);
-    double tot0, tot1;
-    mv(T.wrap, w0, w1, tot0, tot1);
-    double p0 = __shfl(e0, L.src(-1)), p1 = __shfl(e1, L.src(-1));
-    if (c == 0) { p0 = 0.0; p1 = 0.0; }
-    double q0, q1;
-    mv(T.lp[c], tot0, tot1, q0, q1);
-    const double cin0 = p0 + q0, cin1 = p1 + q1;
+    double e[K], tot[K], w[K], cin[K];
+    e[0] = r[M - 1];
+    if (K == 2) e[K - 1] = M > 1 ? r[M > 1 ? M - 2 : 0] : 0.0;
+    wave_scan<K, true>(e, tot, T, c, P);
+    // incoming state of chunk c: the exclusive prefix plus the cyclic total
+    // carried around the line, A^(M c) (I - A^n)^-1 tot
 #pragma unroll
-    for (int k = 0; k < M; k++) r[k] = fma(T.cf[k][0], cin0, fma(T.cf[k][1], cin1, r[k]));
-    if (M > 1) r[M - 2] = fma(a1, r[M - 1], r[M - 2]);
-#pragma unroll
-    for (int k = M - 3; k >= 0; k--) r[k] = fma(a1, r[k + 1], fma(a2, r[k + 2], r[k]));
-    double b0 = r[0], b1 = M > 1 ? r[1] : 0.0;
-    {
-        int j = 0;
-        for (int d = 1; d < P; d <<= 1, j++) {
-            const double t0 = __shfl(b0, L.src(d)), t1 = __shfl(b1, L.src(d));
-            if (c + d < P) {
-                double y0, y1;
-                mv(T.sc[j], t0, t1, y0, y1);
-                b0 += y0;
-                b1 += y1;
-            }
-        }
+    for (int i = 0; i < K; i++) {
+        cin[i] = dpp<DPP_WAVE_SHR1>(e[i]);
+        if (c == 0) cin[i] = 0.0;
+        w[i] = 0.0;
     }
-    w0 = __shfl(b0, L.seg);
-    w1 = __shfl(b1, L.seg);
-    mv(T.wrap, w0, w1, tot0, tot1);
-    p0 = __shfl(b0, L.src(1));
-    p1 = __shfl(b1, L.src(1));
-    if (c == P - 1) { p0 = 0.0; p1 = 0.0; }
-    mv(T.lp[P - 1 - c], tot0, tot1, q0, q1);
-    const double bin0 = p0 + q0, bin1 = p1 + q1;
+    madd<K>(T.wrap, tot, w);
+    madd<K>(T.lp[c], w, cin);
 #pragma unroll
-    for (int k = 0; k < M; k++) r[k] = fma(T.cf[M - 1 - k][0], bin0, fma(T.cf[M - 1 - k][1], bin1, r[k]));
+    for (int k = 0; k < M; k++) {
+        r[k] = fma(T.cf[k][0], cin[0], r[k]);
+        if (K == 2) r[k] = fma(T.cf[k][1], cin[K - 1], r[k]);
+    }
+    // backward recurrence inside the chunk
+#pragma unroll
+    for (int k = M - 2; k >= 0; k--) {
+        r[k] = fma(a1, r[k + 1], r[k]);
+        if (K == 2 && k <= M - 3) r[k] = fma(a2, r[k + 2], r[k]);
+    }
+    e[0] = r[0];
+    if (K == 2) e[K - 1] = M > 1 ? r[M > 1 ? 1 : 0] : 0.0;
+    wave_scan<K, false>(e, tot, T, c, P);
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        cin[i] = dpp<DPP_WAVE_SHL1>(e[i]);
+        if (c == P - 1) cin[i] = 0.0;
+        w[i] = 0.0;
+    }
+    madd<K>(T.wrap, tot, w);
+    madd<K>(T.lp[P - 1 - c], w, cin);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        r[k] = fma(T.cf[M - 1 - k][0], cin[0], r[k]);
+        if (K == 2) r[k] = fma(T.cf[M - 1 - k][1], cin[K - 1], r[k]);
+    }
 }
 
 // ---- operators on a window -----------------------------------------------
@@ -249,7 +350,7 @@ DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
         const int i = k + HALO;
         out[k] = K * ((C.c1 * (w[i + 3] - w[i - 3]) + C.b1 * (w[i + 2] - w[i - 2])) + C.a1 * (w[i + 1] - w[i - 1]));
     }
-    solve1<M>(out, T, L);
+    solve<1, M>(out, T, L);
 }
 
 // Second derivative: K = (N-1)^2*sx(2)*(1+rho^2).
@@ -262,7 +363,7 @@ DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
         const int i = k + HALO;
         out[k] = K * (((C.c2 * (w[i + 3] + w[i - 3]) + C.b2 * (w[i + 2] + w[i - 2])) + C.a2 * (w[i + 1] + w[i - 1])) - C.d2 * w[i]);
     }
-    solve1<M>(out, T, L);
+    solve<1, M>(out, T, L);
 }
 
 // Eighth-order compact filter: K = 1/kappa.
@@ -276,7 +377,7 @@ DEV void op_filt(const double (&w)[M + 2 * HALO], const Coef &C, double K, const
         out[k] = K * ((((C.f4 * (w[i + 4] + w[i - 4]) + C.f3 * (w[i + 3] + w[i - 3])) + C.f2 * (w[i + 2] + w[i - 2]))
                        + C.f1 * (w[i + 1] + w[i - 1])) + C.f0 * w[i]);
     }
-    solve2<M>(out, T, L);
+    solve<2, M>(out, T, L);
 }
 
 // Truncated Gaussian (explicit).
@@ -310,14 +411,17 @@ DEV Prim prim(const Phys &ph, double q1, double q2, double q3, double q4, double
     return r;
 }
 
+// mu = (gamma-1 T)^vis_exp / Re and d mu/dT = vis_exp mu / T
+// (src/rhsv_periodic_sf.f:245-247, the derivative written without a second
+// power; the two forms differ by rounding).
 DEV double visc(const Phys &ph, double T)
 {
-    return pow(ph.gm1 * T, ph.vis_exp) * ph.re_inv;
+    return series_pow(ph.gm1 * T, ph.vis_exp) * ph.re_inv;
 }
 
-DEV double dvisc(const Phys &ph, double T)
+DEV double dvisc(const Phys &ph, double T, double mu)
 {
-    return pow(ph.gm1 * T, ph.vis_exp_m1) * ph.re_inv * ph.vis_exp * ph.gm1;
+    return ph.vis_exp * mu / T;
 }
 
 }  // namespace st

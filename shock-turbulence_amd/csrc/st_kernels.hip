@@ -249,7 +249,7 @@ struct PassGrad {
 #pragma unroll
                 for (int k = 0; k < M; k++) {
                     L.st(8, k, gd[k]);
-                    const double mup = dvisc(ph, f[k]);
+                    const double mup = dvisc(ph, f[k], mu[k]);
                     out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
                 }
             }
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
         const double gxx = G[gi], gyx = G[N + gi], gzx = G[2 * N + gi];
         const double gxy = G[3 * N + gi], gyy = G[4 * N + gi], gzy = G[5 * N + gi];
         const double gxz = G[6 * N + gi], gyz = G[7 * N + gi], gzz = G[8 * N + gi];
-        const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T);
+        const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
         const double tx = mup * G[9 * N + gi], ty = mup * G[10 * N + gi], tz = mup * G[11 * N + gi];
         const double f1 = ph.bulk - 2.0 / 3.0;
         const double dil = gxx + gyy + gzz;

@@ -28,7 +28,10 @@ def _log(*a):
     sys.stdout.flush()
 
 
-SHAPES = [(32, 32, 32), (64, 32, 16), (16, 64, 32)]
+# Line lengths 16..256 cover every lane layout of the passes: lines of
+# P = 8, 16, 32 and 64 lanes (one to eight lines per wave).
+SHAPES = [(32, 32, 32), (64, 32, 16), (16, 64, 32), (256, 16, 16), (16, 128, 32), (16, 16, 256)]
+LONG = [(256, 16, 16), (16, 128, 32), (32, 16, 256)]
 
 
 @pytest.fixture(scope="module")
@@ -83,7 +86,7 @@ def test_d1_sinusoid(solvers, shape):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG)
 def test_rhs(solvers, shape, hyper):
     _log("rhs", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
@@ -135,8 +138,9 @@ def test_reductions_and_fix(solvers):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-def test_stage(solvers, hyper):
-    shape = (32, 32, 32)
+@pytest.mark.parametrize("shape", [(32, 32, 32)] + LONG)
+def test_stage(solvers, shape, hyper):
+    _log("stage", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
     Q = smooth_state(*shape, seed=4)
     QW = np.zeros_like(Q)
