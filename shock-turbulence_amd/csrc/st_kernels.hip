@@ -18,19 +18,24 @@ namespace st {
 // generic pass kernel
 // ---------------------------------------------------------------------
 template <class PASS>
-constexpr int pass_slots() { return PASS::NSLOT + PASS::NEARLY; }
+constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PASS::NEARLY; }
 
 // Slots 0..NSLOT-1 hold the staged inputs; compute() may write final
-// results early into slots NSLOT..NSLOT+NEARLY-1 (freeing their registers);
-// the NOUT register results are staged into slots 0..NOUT-1 after compute.
-// store() receives, per point, the NOUT late results followed by the
-// NEARLY early ones.
+// results early (freeing their registers) into slots NSLOT..NSLOT+NEARLY-1,
+// or, with REUSE, over its own points of the input slots 0..NEARLY-1 once
+// it no longer reads them (each lane reads and writes only its own points
+// of the tile during compute).  The NOUT register results are staged into
+// slots 0..NOUT-1 after compute.  Without REUSE, store() receives per point
+// the NOUT late results followed by the NEARLY early ones; with REUSE the
+// early ones go first through store_early() and store() gets the late ones.
 template <int M, int NT, class PASS>
 __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
 {
     extern __shared__ double lds[];
     constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
+    constexpr bool RE = PASS::REUSE;
     static_assert(NO <= NS, "late outputs must fit in the input slots");
+    static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
     const long tile = tile_of(g);
     const long line0 = tile * g.B;
     const int ss = slot_size<M>(g.P, g.B);
@@ -58,6 +63,21 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
 
     // 3. store (through LDS, memory order)
     __syncthreads();
+    if (RE) {
+#pragma unroll 2
+        for (int r = 0; r < M; r++) {
+            const int e = threadIdx.x + NT * r;
+            int b, s;
+            tile_elem(g, e, b, s);
+            const long gi = gindex(g, line0 + b, s);
+            const int li = lds_idx<M>(b, s % M, s / M, g.P);
+            double v[NE > 0 ? NE : 1];
+#pragma unroll
+            for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
+            PASS::store_early(g, a, gi, v);
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int j = 0; j < NO; j++) {
 #pragma unroll
@@ -72,14 +92,23 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
         const long gi = gindex(g, line0 + b, s);
         const int k = s % M, c = s / M;
         const int li = lds_idx<M>(b, k, c, g.P);
-        double v[NO + NE];
+        constexpr int NV = RE ? NO : NO + NE;
+        double v[NV];
 #pragma unroll
         for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
+        if (!RE) {
 #pragma unroll
-        for (int j = 0; j < NE; j++) v[NO + j] = lds[(NS + j) * ss + li];
+            for (int j = NO; j < NV; j++) v[j] = lds[(NS + j - NO) * ss + li];
+        }
         PASS::store(g, a, gi, v);
     }
 }
+
+// Passes whose early outputs have their own slots.
+struct NoReuse {
+    static constexpr bool REUSE = false;
+    DEV static void store_early(const Geo &, const PassArgs &, long, const double *) {}
+};
 
 // ---------------------------------------------------------------------
 // helpers
@@ -105,7 +134,7 @@ DEV void upd(double *p, double v, bool first)
 // single-operator passes (the DFDX/D2FDX2/Compact_Filter_X/Gaussian_X
 // entry points of the C-ABI)
 // ---------------------------------------------------------------------
-struct PassD1 {
+struct PassD1 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
@@ -123,7 +152,7 @@ struct PassD1 {
     }
 };
 
-struct PassD2 {
+struct PassD2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 1;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
@@ -149,7 +178,7 @@ struct PassD2 {
     }
 };
 
-struct PassFilt {
+struct PassFilt : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
@@ -164,7 +193,7 @@ struct PassFilt {
     DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v) { a.out0[gi] = v[0]; }
 };
 
-struct PassGauss {
+struct PassGauss : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
     template <int M>
@@ -191,13 +220,14 @@ struct PassGauss {
 //   dQ_4 += -D1(rho u_d)
 //   dQ_5 += D1(-(rho T + rho |u|^2/2) u_d) + sum_i u_i w_id mu d2u_i/dx_d^2
 //           + Pr^-1 (mu d2T/dx_d^2 + mu' (dT/dx_d)^2)
-//   G(., d) = D1(u, v, w, T);  L2e += D2(T/gamma)   (e = T/gamma)
-// D1 of a sum replaces the reference's sum of D1s of the same operator, and
-// rho*E + P = rho T + rho |u|^2 / 2; both differ from the reference's
-// evaluation only by rounding.
+//   G(., d) = D1(u, v, w, T);  L2e += D2(T)/gamma   (e = T/gamma)
+// D1 of a sum replaces the reference's sum of D1s of the same operator,
+// rho*E + P = rho T + rho |u|^2 / 2, and D2(T)/gamma replaces D2(T/gamma);
+// all three differ from the reference's evaluation only by rounding.
 // ---------------------------------------------------------------------
 struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
+    static constexpr bool REUSE = true;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
@@ -214,71 +244,22 @@ struct PassGrad {
 #pragma unroll
         for (int r = 0; r < 5; r++) out[r][k] += (i == r) ? v : 0.0;
     }
+    // Operator order: fluxes and the mass flux first, while u, v, w, T and
+    // rho are all needed; then T, whose slot takes dT/dx_d and rho's slot
+    // the Laplacian of e; then each velocity, whose slot takes its
+    // derivative.
     template <int M>
     DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
-        double mu[M];
+        double mu[M], rud[M];
 #pragma unroll
         for (int k = 0; k < M; k++) {
             mu[k] = visc(ph, L.ld(3, k));
+            rud[k] = -0.5 * L.ld(4, k) * L.ld(d, k);
 #pragma unroll
             for (int i = 0; i < 5; i++) out[i][k] = 0.0;
-        }
-        // u, v, w, T: first and second derivatives (one operator pair at a time)
-#pragma unroll 1
-        for (int comp = 0; comp < 4; comp++) {
-            double f[M], w[M + 2 * HALO], gd[M], hd[M];
-#pragma unroll
-            for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
-            window<M>(f, L, w);
-            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
-            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
-            if (comp < 3) {
-                const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    L.st(5 + comp, k, gd[k]);
-                    const double rho = L.ld(4, k), ud = L.ld(d, k);
-                    const double vt = wd * mu[k] * hd[k];
-                    acc<M>(out, comp, k, vt + (-0.5 * rho * ud) * gd[k]);
-                    out[4][k] += f[k] * vt;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    L.st(8, k, gd[k]);
-                    const double mup = dvisc(ph, f[k], mu[k]);
-                    out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
-                }
-            }
-        }
-        // internal energy e = T/gamma: Laplacian contribution
-        {
-            double f[M], w[M + 2 * HALO], le[M];
-#pragma unroll
-            for (int k = 0; k < M; k++) f[k] = L.ld(3, k) * ph.ginv;
-            window<M>(f, L, w);
-            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, le);
-#pragma unroll
-            for (int k = 0; k < M; k++) L.st(9, k, le[k]);
-        }
-        // skew-symmetric part and continuity with the mass flux rho u_d
-        {
-            double f[M], w[M + 2 * HALO], sk[M];
-#pragma unroll
-            for (int k = 0; k < M; k++) f[k] = L.ld(4, k) * L.ld(d, k);
-            window<M>(f, L, w);
-            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, sk);
-#pragma unroll
-            for (int k = 0; k < M; k++) {
-                const double t = -0.5 * sk[k];
-                out[0][k] += t * L.ld(0, k);
-                out[1][k] += t * L.ld(1, k);
-                out[2][k] += t * L.ld(2, k);
-                out[3][k] = -sk[k];
-            }
         }
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
@@ -304,15 +285,64 @@ struct PassGrad {
 #pragma unroll
             for (int k = 0; k < M; k++) acc<M>(out, comp, k, df[k]);
         }
+        // skew-symmetric part and continuity with the mass flux rho u_d
+        {
+            double f[M], w[M + 2 * HALO], sk[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(4, k) * L.ld(d, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, sk);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double t = -0.5 * sk[k];
+                out[0][k] += t * L.ld(0, k);
+                out[1][k] += t * L.ld(1, k);
+                out[2][k] += t * L.ld(2, k);
+                out[3][k] = -sk[k];
+            }
+        }
+        // T, then u, v, w: first and second derivatives.  The Laplacian of
+        // e = T/gamma is D2(T)/gamma (the operator is linear).
+#pragma unroll 1
+        for (int it = 0; it < 4; it++) {
+            const int comp = (it + 3) & 3;      // 3, 0, 1, 2
+            double f[M], w[M + 2 * HALO], gd[M], hd[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
+            if (comp < 3) {
+                const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    L.st(comp, k, gd[k]);
+                    const double vt = wd * mu[k] * hd[k];
+                    acc<M>(out, comp, k, vt + rud[k] * gd[k]);
+                    out[4][k] += f[k] * vt;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    L.st(3, k, gd[k]);
+                    L.st(4, k, hd[k] * ph.ginv);
+                    const double mup = dvisc(ph, f[k], mu[k]);
+                    out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
+                }
+            }
+        }
     }
-    DEV static void store(const Geo &g, const PassArgs &a, long gi, const double *v)
+    DEV static void store_early(const Geo &g, const PassArgs &a, long gi, const double *v)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.G[(g.dir * 3 + c) * a.N + gi] = v[c];
+        a.G[(9 + g.dir) * a.N + gi] = v[3];
+        if (a.L2e) upd(&a.L2e[gi], v[4], a.first);
+    }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
 #pragma unroll
         for (int c = 0; c < 5; c++) upd(&a.dQ[c * a.N + gi], v[c], a.first);
-#pragma unroll
-        for (int c = 0; c < 3; c++) a.G[(g.dir * 3 + c) * a.N + gi] = v[5 + c];
-        a.G[(9 + g.dir) * a.N + gi] = v[8];
-        if (a.L2e) upd(&a.L2e[gi], v[9], a.first);
     }
 };
 
@@ -325,7 +355,7 @@ struct PassGrad {
 //   z pass: D1z(dux/dx), D1z(duz/dx)
 // ---------------------------------------------------------------------
 template <int DIR, bool HYP>
-struct PassMix {
+struct PassMix : NoReuse {
     static constexpr int NMIX = DIR == 1 ? 4 : DIR == 2 ? 2 : 0;
     static constexpr int NH = HYP ? 2 : 0;
     static constexpr int NSLOT = NMIX + NH, NOUT = NMIX + NH, NEARLY = 0;
@@ -391,7 +421,7 @@ struct PassMix {
 // (src/hyper_dynamic.f:278-297).
 // ---------------------------------------------------------------------
 template <bool LAST>
-struct PassLap2 {
+struct PassLap2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
     template <int M>
@@ -425,7 +455,7 @@ struct PassLap2 {
 // transport coefficients beta*, mu*, kappa* (src/hyper_dynamic.f:303-314).
 // ---------------------------------------------------------------------
 template <bool LAST>
-struct PassGaussEta {
+struct PassGaussEta : NoReuse {
     static constexpr int NSLOT = 2, NOUT = 2, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[2])
     {
@@ -466,7 +496,7 @@ struct PassGaussEta {
 // D1 of each energy term separately; the sum of the D1s is the D1 of the
 // sum).
 // ---------------------------------------------------------------------
-struct PassHyperDiv {
+struct PassHyperDiv : NoReuse {
     static constexpr int NSLOT = 4, NOUT = 4, NEARLY = 0;
     DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
     {
@@ -525,7 +555,7 @@ struct PassHyperDiv {
 // z pass applies Fix_Pressure (src/lddrk4_periodic_sf.f:436-473).
 // ---------------------------------------------------------------------
 template <int KIND>   // 0: RK update + x filter, 1: y filter, 2: z filter + pressure fix
-struct PassFilterQ {
+struct PassFilterQ : NoReuse {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {

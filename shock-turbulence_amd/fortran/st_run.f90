@@ -1,0 +1,101 @@
+! st_run.f90 -- march-mode-1 driver of the periodic solver on one MI355X,
+! following Program Main (src/main.f:99-152): set up, initial condition,
+! Compute_CFL / Compute_Tstep, then per step Compute_CFL, Compute_Tstep,
+! LDDRK4 and one line of the 'output' log.
+!
+! Input: a namelist file named by the first argument (defaults below).
+! The initial condition is the seeded isotropic field of st_ic_isotropic
+! (the reference restarts from a file instead, src/main.f:69-71).  With
+! Re <= 0 the Reynolds number is set from re_lambda at t = 0:
+! Re = (1/M_t^2)^vis_exp / (u_rms (2/k0) / Re_lambda).
+!
+! Output: the log columns of src/main.f:109-116 that this build computes
+! (MaxDil and Enstrophy of Compute_Grad are not), in the reference format
+! (1x,i5,ES13.5) appended to the file 'output', and the same values at full
+! precision on standard output.  With qout set, the final state is written
+! there as raw FP64 Q(My,Mz,Mx,5).
+program st_run
+  use st_capi
+  use st_state
+  implicit none
+  integer :: nx = 64, ny = 64, nz = 64, nsteps = 10, hyper_param = 1, d2_mode = 0, rk_scheme = 0
+  integer(8) :: seed = 12345
+  real(8) :: k0 = 4.0d0, urms = 1.0d0, mt = 0.3d0, re_lambda = 40.0d0, Re = 0.0d0, Pr = 0.7d0
+  real(8) :: gamma = 1.4d0, vis_exp = 0.76d0, bulk_ratio = 0.0d0, cfl_cal = 0.5d0
+  real(8) :: period = 6.283185307179586d0
+  character(len=256) :: qout = ' '
+  namelist /run/ nx, ny, nz, nsteps, hyper_param, d2_mode, rk_scheme, seed, k0, urms, mt, &
+       re_lambda, Re, Pr, gamma, vis_exp, bulk_ratio, cfl_cal, period, qout
+
+  type(st_grid) :: grid
+  type(st_params) :: params
+  real(8), allocatable :: Q(:, :, :, :)
+  real(8) :: time, cfl_stab, cfl_resv, per(3)
+  real(8) :: MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy
+  integer :: step, step_count, u, ios
+  character(len=256) :: arg
+
+  if (command_argument_count() >= 1) then
+     call get_command_argument(1, arg)
+     open(newunit=u, file=trim(arg), status='old', action='read', iostat=ios)
+     if (ios /= 0) then
+        write(*, '(a,a)') 'st_run: cannot open ', trim(arg)
+        error stop 2
+     end if
+     read(u, nml=run)
+     close(u)
+  end if
+  if (Re <= 0.0d0) Re = (1.0d0 / (mt * mt)) ** vis_exp / (urms * (2.0d0 / k0) / re_lambda)
+
+  grid = st_grid(nx, ny, nz, period, period, period, 0.0d0, 0.0d0, 0.0d0)
+  params = st_params(Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal, d2_mode, rk_scheme)
+  call Setup_Solver(grid, params)
+
+  allocate(Q(ny, nz, nx, 5))
+  per = period
+  call st_check(st_ic_isotropic(nx, ny, nz, per, k0, urms, mt, gamma, seed, Q), 'st_ic_isotropic')
+  call Upload_State(Q)
+
+  time = 0.0d0
+  step = 0
+  call Compute_CFL(Q, cfl_stab, cfl_resv, step)
+  call Compute_Tstep(Q, cfl_resv, step)
+  call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
+  call log_header()
+  call log_line()
+
+  do step_count = 1, nsteps
+     call Compute_CFL(Q, cfl_stab, cfl_resv, step)
+     call Compute_Tstep(Q, cfl_resv, step)
+     call LDDRK4(time, step)
+     call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
+     call log_line()
+  end do
+
+  if (len_trim(qout) > 0) then
+     call Download_State(Q)
+     open(newunit=u, file=trim(qout), access='stream', form='unformatted', status='replace')
+     write(u) Q
+     close(u)
+  end if
+  call Finalize_Solver()
+
+contains
+
+  subroutine log_header()
+    open(newunit=u, file='output', form='formatted', status='unknown', position='append')
+    write(u, '(1x,a5,10(a13))') 'n ', 't   ', 'dt    ', 'CFL_resv  ', 'KE     ', &
+         'MomX  ', 'MomY  ', 'MomZ  ', 'Mass  ', 'Energy ', 'Entropy'
+    close(u)
+  end subroutine log_header
+
+  subroutine log_line()
+    open(newunit=u, file='output', form='formatted', status='unknown', position='append')
+    write(u, '(1x,i5,10(ES13.5))') step, time, Tstep, cfl_resv * Tstep, KE, &
+         MomX, MomY, MomZ, Mass_t, e_t, Entropy
+    close(u)
+    write(*, '(a,i6,10(1x,ES24.16E3))') 'step', step, time, Tstep, cfl_resv * Tstep, KE, &
+         MomX, MomY, MomZ, Mass_t, e_t, Entropy
+  end subroutine log_line
+
+end program st_run

@@ -1,0 +1,53 @@
+"""The Fortran driver (shock-turbulence_amd/fortran): the reference's
+subroutine names over the C-ABI, run as the reference's march-mode-1 main
+program, against the oracle's march on the same initial condition."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fields import relerr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "shock-turbulence_amd", "fortran", "st_run")
+
+needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (run build())")
+
+
+@needs_exe
+def test_driver_reports_missing_input(tmp_path):
+    r = subprocess.run([EXE, str(tmp_path / "absent.nml")], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "cannot open" in r.stdout
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_driver_march_matches_oracle(tmp_path):
+    st_amd = pytest.importorskip("st_amd")
+    n, nsteps, seed = 32, 5, 4242
+    k0, urms, mt = 4.0, 1.0, 0.3
+    nml = tmp_path / "run.nml"
+    nml.write_text("&run nx=%d, ny=%d, nz=%d, nsteps=%d, seed=%d, k0=%g, urms=%g, mt=%g, "
+                   "re_lambda=40, pr=0.7, hyper_param=1, qout='q.bin' /\n" % (n, n, n, nsteps, seed, k0, urms, mt))
+    print("fortran driver", n, nsteps, flush=True)
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l.split() for l in r.stdout.splitlines() if l.startswith("step")]
+    assert len(lines) == nsteps + 1
+    Qg = np.fromfile(tmp_path / "q.bin", dtype=np.float64).reshape((n, n, n, 5), order="F")
+    log = (tmp_path / "output").read_text().splitlines()
+    assert len(log) == nsteps + 2 and log[0].split()[0] == "n"
+
+    re = st_amd.re_for_taylor(40.0, k0, urms, mt)
+    o = O.Oracle(n, n, n, Re=re, Pr=0.7, hyper_param=1)
+    Q0 = st_amd.ic_isotropic(n, n, n, k0=k0, urms=urms, mt=mt, seed=seed)
+    Qc, t_c, st_c, dts = o.march(Q0, nsteps)
+    assert relerr(Qg, Qc).max() < 1e-10
+    last = [float(x) for x in lines[-1][2:]]
+    assert int(lines[-1][1]) == st_c == nsteps
+    assert abs(last[0] - t_c) <= 1e-12 * t_c
+    assert abs(last[1] - dts[-1]) <= 1e-11 * dts[-1]
+    np.testing.assert_allclose(last[3:], o.diagnostics(Qc)[[3, 0, 1, 2, 4, 5, 6]], rtol=1e-10, atol=1e-13)
