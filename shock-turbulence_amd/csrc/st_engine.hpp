@@ -94,8 +94,10 @@ struct Lane {
     DEV int src(int dc) const { return seg + ((c + dc) & (P - 1)); }
 };
 
+// ss = B*M*P = NT*M doubles: a compile-time constant for the kernel, so the
+// slot offsets fold into the LDS instructions' immediate offsets.
 template <int M>
-DEV Lane<M> make_lane(const Geo &g, double *lds)
+DEV Lane<M> make_lane(const Geo &g, double *lds, int ss)
 {
     Lane<M> L;
     const int lane = threadIdx.x & 63;
@@ -105,7 +107,7 @@ DEV Lane<M> make_lane(const Geo &g, double *lds)
     L.seg = lane - L.c;
     L.b = wave * (64 / g.P) + lane / g.P;
     L.lds = lds;
-    L.ss = slot_size<M>(g.P, g.B);
+    L.ss = ss;
     return L;
 }
 

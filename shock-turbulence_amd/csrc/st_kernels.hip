@@ -29,16 +29,18 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // the NOUT late results followed by the NEARLY early ones; with REUSE the
 // early ones go first through store_early() and store() gets the late ones.
 template <int M, int NT, class PASS>
-__global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
+k_pass(Geo g, PassArgs a)
 {
     extern __shared__ double lds[];
     constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
     constexpr bool RE = PASS::REUSE;
     static_assert(NO <= NS, "late outputs must fit in the input slots");
+    static_assert(NO > 0 || RE, "a pass without late outputs stores through store_early");
     static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
     const long tile = tile_of(g);
     const long line0 = tile * g.B;
-    const int ss = slot_size<M>(g.P, g.B);
+    constexpr int ss = NT * M;           // slot_size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
 #pragma unroll 2
@@ -57,8 +59,8 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
     __syncthreads();
 
     // 2. compute (line mapping)
-    const Lane<M> L = make_lane<M>(g, lds);
-    double out[NO][M];
+    const Lane<M> L = make_lane<M>(g, lds, ss);
+    double out[NO > 0 ? NO : 1][M];
     PASS::template compute<M>(g, a, L, out);
 
     // 3. store (through LDS, memory order)
@@ -76,6 +78,7 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
             PASS::store_early(g, a, gi, v);
         }
+        if constexpr (NO == 0) return;
         __syncthreads();
     }
 #pragma unroll
@@ -93,7 +96,7 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
         const int k = s % M, c = s / M;
         const int li = lds_idx<M>(b, k, c, g.P);
         constexpr int NV = RE ? NO : NO + NE;
-        double v[NV];
+        double v[NV > 0 ? NV : 1];
 #pragma unroll
         for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
         if (!RE) {
@@ -107,6 +110,7 @@ __global__ void __launch_bounds__(NT) k_pass(Geo g, PassArgs a)
 // Passes whose early outputs have their own slots.
 struct NoReuse {
     static constexpr bool REUSE = false;
+    static constexpr int MIN_WAVES = 1;
     DEV static void store_early(const Geo &, const PassArgs &, long, const double *) {}
 };
 
@@ -228,6 +232,7 @@ struct PassGauss : NoReuse {
 struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = 3;   // 168 registers: 3 waves per SIMD
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
@@ -237,22 +242,26 @@ struct PassGrad {
         v[3] = pr.T;
         v[4] = a.Q[3 * a.N + gi];
     }
-    // Accumulate into row i of out with a static register index.
-    template <int M>
+    // Accumulate into row i of out (one of the rows in MASK) with static
+    // register indices.
+    template <int MASK, int M>
     DEV static void acc(double (&out)[5][M], int i, int k, double v)
     {
 #pragma unroll
-        for (int r = 0; r < 5; r++) out[r][k] += (i == r) ? v : 0.0;
+        for (int r = 0; r < 5; r++)
+            if ((MASK >> r) & 1) out[r][k] += (i == r) ? v : 0.0;
     }
     // Operator order: fluxes and the mass flux first, while u, v, w, T and
     // rho are all needed; then T, whose slot takes dT/dx_d and rho's slot
     // the Laplacian of e; then each velocity, whose slot takes its
-    // derivative.
+    // derivative.  The loop bodies are free of branches so that the
+    // accumulators stay in place across iterations.
     template <int M>
     DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
+        const double pfac = ph.gm1 * ph.ginv;
         double mu[M], rud[M];
 #pragma unroll
         for (int k = 0; k < M; k++) {
@@ -263,27 +272,23 @@ struct PassGrad {
         }
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
-        for (int comp = 0; comp < 5; comp++) {
-            if (comp == 3) continue;
+        for (int it = 0; it < 4; it++) {
+            const int comp = it == 3 ? 4 : it;
+            const int ci = it == 3 ? 0 : it;
+            const double pw = comp == d ? 1.0 : 0.0;
             double f[M], w[M + 2 * HALO], df[M];
 #pragma unroll
             for (int k = 0; k < M; k++) {
                 const double u = L.ld(0, k), v = L.ld(1, k), ww = L.ld(2, k), T = L.ld(3, k), rho = L.ld(4, k);
-                const double ud = L.ld(d, k);
-                double fl;
-                if (comp < 3) {
-                    const double ui = L.ld(comp, k);
-                    fl = -0.5 * rho * ui * ud;
-                    if (comp == d) fl -= rho * T * ph.gm1 * ph.ginv;
-                } else {
-                    fl = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
-                }
-                f[k] = fl;
+                const double ud = L.ld(d, k), ui = L.ld(ci, k);
+                const double fm = -0.5 * rho * ui * ud - pw * (rho * T * pfac);
+                const double fe = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
+                f[k] = it == 3 ? fe : fm;
             }
             window<M>(f, L, w);
             op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, df);
 #pragma unroll
-            for (int k = 0; k < M; k++) acc<M>(out, comp, k, df[k]);
+            for (int k = 0; k < M; k++) acc<0x17, M>(out, comp, k, df[k]);
         }
         // skew-symmetric part and continuity with the mass flux rho u_d
         {
@@ -301,34 +306,39 @@ struct PassGrad {
                 out[3][k] = -sk[k];
             }
         }
-        // T, then u, v, w: first and second derivatives.  The Laplacian of
-        // e = T/gamma is D2(T)/gamma (the operator is linear).
+        // T: first and second derivatives; the Laplacian of e = T/gamma is
+        // D2(T)/gamma (the operator is linear)
+        {
+            double f[M], w[M + 2 * HALO], gd[M], hd[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(3, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                L.st(3, k, gd[k]);
+                L.st(4, k, hd[k] * ph.ginv);
+                const double mup = dvisc(ph, f[k], mu[k]);
+                out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
+            }
+        }
+        // u, v, w: first and second derivatives
 #pragma unroll 1
-        for (int it = 0; it < 4; it++) {
-            const int comp = (it + 3) & 3;      // 3, 0, 1, 2
+        for (int comp = 0; comp < 3; comp++) {
             double f[M], w[M + 2 * HALO], gd[M], hd[M];
 #pragma unroll
             for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
             window<M>(f, L, w);
             op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
             op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
-            if (comp < 3) {
-                const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
+            const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
 #pragma unroll
-                for (int k = 0; k < M; k++) {
-                    L.st(comp, k, gd[k]);
-                    const double vt = wd * mu[k] * hd[k];
-                    acc<M>(out, comp, k, vt + rud[k] * gd[k]);
-                    out[4][k] += f[k] * vt;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    L.st(3, k, gd[k]);
-                    L.st(4, k, hd[k] * ph.ginv);
-                    const double mup = dvisc(ph, f[k], mu[k]);
-                    out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
-                }
+            for (int k = 0; k < M; k++) {
+                L.st(comp, k, gd[k]);
+                const double vt = wd * mu[k] * hd[k];
+                acc<0x07, M>(out, comp, k, vt + rud[k] * gd[k]);
+                out[4][k] += f[k] * vt;
             }
         }
     }
@@ -355,10 +365,12 @@ struct PassGrad {
 //   z pass: D1z(dux/dx), D1z(duz/dx)
 // ---------------------------------------------------------------------
 template <int DIR, bool HYP>
-struct PassMix : NoReuse {
+struct PassMix {
     static constexpr int NMIX = DIR == 1 ? 4 : DIR == 2 ? 2 : 0;
     static constexpr int NH = HYP ? 2 : 0;
-    static constexpr int NSLOT = NMIX + NH, NOUT = NMIX + NH, NEARLY = 0;
+    static constexpr int NSLOT = NMIX + NH, NOUT = 0, NEARLY = NMIX + NH;
+    static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = 1;
     // G index of each mixed input: gxx=0, gyx=1, gzx=2, gxy=3, gyy=4, gzy=5, gxz=6, gyz=7, gzz=8
     DEV static int gsrc(int m)
     {
@@ -375,19 +387,32 @@ struct PassMix : NoReuse {
         }
     }
     template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[NOUT][M])
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
     {
-#pragma unroll
-        for (int m = 0; m < NSLOT; m++) {
-            double f[M], w[M + 2 * HALO];
+        // each result replaces its own input in LDS
+#pragma unroll 1
+        for (int m = 0; m < NMIX; m++) {
+            double f[M], w[M + 2 * HALO], r[M];
 #pragma unroll
             for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
             window<M>(f, L, w);
-            if (m < NMIX) op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, out[m]);
-            else op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[m]);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
+        }
+#pragma unroll 1
+        for (int m = NMIX; m < NSLOT; m++) {
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
+            window<M>(f, L, w);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         if (NMIX > 0) {
             const Phys &ph = a.ph;
@@ -496,8 +521,10 @@ struct PassGaussEta : NoReuse {
 // D1 of each energy term separately; the sum of the D1s is the D1 of the
 // sum).
 // ---------------------------------------------------------------------
-struct PassHyperDiv : NoReuse {
-    static constexpr int NSLOT = 4, NOUT = 4, NEARLY = 0;
+struct PassHyperDiv {
+    static constexpr int NSLOT = 4, NOUT = 0, NEARLY = 4;
+    static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = 1;
     DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
     {
         const long N = a.N;
@@ -529,18 +556,21 @@ struct PassHyperDiv : NoReuse {
         v[3] = uh * tx + vh * ty + wh * tz + kap * G[(9 + g.dir) * N + gi];
     }
     template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[4][M])
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
     {
-#pragma unroll
+#pragma unroll 1
         for (int m = 0; m < 4; m++) {
-            double f[M], w[M + 2 * HALO];
+            double f[M], w[M + 2 * HALO], r[M];
 #pragma unroll
             for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
             window<M>(f, L, w);
-            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, out[m]);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         a.dQ[gi] += v[0];
         a.dQ[a.N + gi] += v[1];
@@ -555,8 +585,10 @@ struct PassHyperDiv : NoReuse {
 // z pass applies Fix_Pressure (src/lddrk4_periodic_sf.f:436-473).
 // ---------------------------------------------------------------------
 template <int KIND>   // 0: RK update + x filter, 1: y filter, 2: z filter + pressure fix
-struct PassFilterQ : NoReuse {
-    static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 0;
+struct PassFilterQ {
+    static constexpr int NSLOT = 5, NOUT = 0, NEARLY = 5;
+    static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = 1;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
 #pragma unroll
@@ -572,19 +604,22 @@ struct PassFilterQ : NoReuse {
         }
     }
     template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[5][M])
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
     {
         const OpTab &T = a.tab->op[2];
-#pragma unroll
+#pragma unroll 1
         for (int m = 0; m < 5; m++) {
-            double f[M], w[M + 2 * HALO];
+            double f[M], w[M + 2 * HALO], r[M];
 #pragma unroll
             for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
             window<M>(f, L, w);
-            op_filt<M>(w, a.C, T.scale, T, L, out[m]);
+            op_filt<M>(w, a.C, T.scale, T, L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
         }
     }
-    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
     {
         const long N = a.N;
         if (KIND == 2 && a.fix) {
