@@ -45,6 +45,7 @@ struct st_ctx {
     st::DirTab *td = nullptr;
     double *buf = nullptr;
     double *Q, *QW, *dQ, *G, *L2e, *S, *Cb, *L2S, *L4e, *L4S, *eta, *Hc, *T5, *tB, *tC;
+    double *dQp, *L2ep;             // z and x partials of the first-derivative passes
     double *red = nullptr;          // reduction partials (device)
     double *red_h = nullptr;        // pinned host copy
     hipStream_t s = nullptr;
@@ -136,6 +137,8 @@ st::PassArgs base_args(st_ctx *c)
     a.L4S = c->L4S;
     a.eta = c->eta;
     a.Hc = c->Hc;
+    a.dQp = c->dQp;
+    a.L2ep = c->L2ep;
     return a;
 }
 
@@ -161,7 +164,7 @@ double kind_fields(int kind, const st::PassArgs &a)
     case st::K_RKF_X: return 15 + 5 + 5;
     case st::K_F_Y: return 10;
     case st::K_F_Z: return 10;
-    case KIND_CROSS: return 12 + 5 + 8 + 2;
+    case KIND_CROSS: return 12 + 5 + 15 + 5 + (a.L2e ? 2 + 4 : 0);
     default: return 0;
     }
 }
@@ -308,7 +311,7 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
         *out = c;
         return -3;
     }
-    const long nf = 5 + 5 + 5 + 12 + 1 + 1 + 1 + 1 + 1 + 1 + 2 + 3 + 5 + 1 + 1;
+    const long nf = 5 + 5 + 5 + 12 + 1 + 1 + 1 + 1 + 1 + 1 + 2 + 3 + 5 + 1 + 1 + 10 + 2;
     if ((e = hipMalloc(&c->buf, sizeof(double) * nf * c->N)) != hipSuccess ||
         (e = hipMalloc(&c->td, sizeof(st::DirTab) * 6)) != hipSuccess ||
         (e = hipMalloc(&c->red, sizeof(double) * 1024 * 8)) != hipSuccess ||
@@ -334,6 +337,8 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     c->T5 = p; p += 5 * N;
     c->tB = p; p += N;
     c->tC = p; p += N;
+    c->dQp = p; p += 10 * N;
+    c->L2ep = p; p += 2 * N;
     if ((e = hipMemcpyAsync(c->td, c->th, sizeof(st::DirTab) * 6, hipMemcpyHostToDevice, c->s)) != hipSuccess ||
         (e = st::run_fill(c->buf, nf * N, 0.0, c->s)) != hipSuccess ||
         (e = hipStreamSynchronize(c->s)) != hipSuccess) {
@@ -438,9 +443,18 @@ int seq_rhs(st_ctx *c)
     st::PassArgs a = base_args(c);
     a.first = 1;
     TRY(pass(c, st::K_GRAD, 1, a));
+    // the z and x passes write their own partial sums (no read-modify-write);
+    // the cross kernel adds them up
+    {
+        st::PassArgs b = a;
+        b.dQ = c->dQp;
+        b.L2e = hyp ? c->L2ep : nullptr;
+        TRY(pass(c, st::K_GRAD, 2, b));
+        b.dQ = c->dQp + 5 * c->N;
+        b.L2e = hyp ? c->L2ep + c->N : nullptr;
+        TRY(pass(c, st::K_GRAD, 0, b));
+    }
     a.first = 0;
-    TRY(pass(c, st::K_GRAD, 2, a));
-    TRY(pass(c, st::K_GRAD, 0, a));
     {
         Event e;
         if (c->prof) prof_begin(c, KIND_CROSS * 3, e);

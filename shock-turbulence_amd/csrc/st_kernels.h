@@ -19,6 +19,8 @@ struct PassArgs {
     double *L2e, *S, *Cb, *L2S, *L4e, *L4S;
     double *eta;          // 2 fields
     double *Hc;           // beta*, mu*, kappa*
+    const double *dQp;    // z and x right-hand-side partials of the first-derivative passes (2 x 5 fields)
+    const double *L2ep;   // z and x partials of the Laplacian of e (2 fields)
     const double *in0;    // single-operator input
     double *out0, *out1;  // single-operator outputs
     int iadd;             // single-operator accumulate flag

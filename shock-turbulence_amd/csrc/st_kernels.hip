@@ -28,6 +28,10 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // slots 0..NOUT-1 after compute.  Without REUSE, store() receives per point
 // the NOUT late results followed by the NEARLY early ones; with REUSE the
 // early ones go first through store_early() and store() gets the late ones.
+// A pass may also name NPF values per point that its store phase reads
+// (read-modify-write operands): prefetch() loads them during the load phase,
+// which visits the same points per thread as the store phase, and they
+// reach store_early() (REUSE) or store() appended after the results.
 template <int M, int NT, class PASS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
 k_pass(Geo g, PassArgs a)
@@ -38,12 +42,15 @@ k_pass(Geo g, PassArgs a)
     static_assert(NO <= NS, "late outputs must fit in the input slots");
     static_assert(NO > 0 || RE, "a pass without late outputs stores through store_early");
     static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
+    constexpr int NPF = PASS::NPF;
+    static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
+    double pf[M][NPF > 0 ? NPF : 1];
     const long tile = tile_of(g);
     const long line0 = tile * g.B;
     constexpr int ss = NT * M;           // slot_size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
-#pragma unroll 2
+#pragma unroll(PASS::LOAD_UNROLL)
     for (int r = 0; r < M; r++) {
         const int e = threadIdx.x + NT * r;
         int b, s;
@@ -51,6 +58,7 @@ k_pass(Geo g, PassArgs a)
         const long gi = gindex(g, line0 + b, s);
         double v[NS > 0 ? NS : 1];
         PASS::load(g, a, gi, v);
+        if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
         const int li = lds_idx<M>(b, k, c, g.P);
 #pragma unroll
@@ -66,16 +74,18 @@ k_pass(Geo g, PassArgs a)
     // 3. store (through LDS, memory order)
     __syncthreads();
     if (RE) {
-#pragma unroll 2
+#pragma unroll
         for (int r = 0; r < M; r++) {
             const int e = threadIdx.x + NT * r;
             int b, s;
             tile_elem(g, e, b, s);
             const long gi = gindex(g, line0 + b, s);
             const int li = lds_idx<M>(b, s % M, s / M, g.P);
-            double v[NE > 0 ? NE : 1];
+            double v[NE + NPF > 0 ? NE + NPF : 1];
 #pragma unroll
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
+#pragma unroll
+            for (int j = 0; j < NPF; j++) v[NE + j] = pf[r][j];
             PASS::store_early(g, a, gi, v);
         }
         if constexpr (NO == 0) return;
@@ -87,7 +97,7 @@ k_pass(Geo g, PassArgs a)
         for (int k = 0; k < M; k++) L.st(j, k, out[j][k]);
     }
     __syncthreads();
-#pragma unroll 2
+#pragma unroll
     for (int r = 0; r < M; r++) {
         const int e = threadIdx.x + NT * r;
         int b, s;
@@ -96,13 +106,15 @@ k_pass(Geo g, PassArgs a)
         const int k = s % M, c = s / M;
         const int li = lds_idx<M>(b, k, c, g.P);
         constexpr int NV = RE ? NO : NO + NE;
-        double v[NV > 0 ? NV : 1];
+        double v[NV + NPF > 0 ? NV + NPF : 1];
 #pragma unroll
         for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
         if (!RE) {
 #pragma unroll
             for (int j = NO; j < NV; j++) v[j] = lds[(NS + j - NO) * ss + li];
         }
+#pragma unroll
+        for (int j = 0; j < NPF; j++) v[NV + j] = pf[r][j];
         PASS::store(g, a, gi, v);
     }
 }
@@ -111,6 +123,9 @@ k_pass(Geo g, PassArgs a)
 struct NoReuse {
     static constexpr bool REUSE = false;
     static constexpr int MIN_WAVES = 1;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 0;
+    DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     DEV static void store_early(const Geo &, const PassArgs &, long, const double *) {}
 };
 
@@ -233,6 +248,9 @@ struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
     static constexpr int MIN_WAVES = 3;   // 168 registers: 3 waves per SIMD
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 0;
+    DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
@@ -370,7 +388,33 @@ struct PassMix {
     static constexpr int NH = HYP ? 2 : 0;
     static constexpr int NSLOT = NMIX + NH, NOUT = 0, NEARLY = NMIX + NH;
     static constexpr bool REUSE = true;
-    static constexpr int MIN_WAVES = 1;
+    // z passes run 512-thread tiles: 4 waves per SIMD keeps two resident
+    static constexpr int MIN_WAVES = DIR == 2 ? 4 : 1;
+    static constexpr int LOAD_UNROLL = 8;
+    // prefetched for the store phase: (1/3 + bulk) mu, u, v, w and the
+    // right-hand sides 1, 3, 5 (and 2 for y); L2S and L4e when they are
+    // accumulated (not the first mix pass)
+    static constexpr int NPQ = NMIX > 0 ? 4 + (DIR == 1 ? 4 : 3) : 0;
+    static constexpr int NPF = NPQ + (HYP && DIR != 0 ? 2 : 0);
+    DEV static void prefetch(const Geo &, const PassArgs &a, long gi, double *p)
+    {
+        if (NMIX > 0) {
+            const Phys &ph = a.ph;
+            const Prim pr = prim(ph, qf(a, 0, gi), qf(a, 1, gi), qf(a, 2, gi), qf(a, 3, gi), qf(a, 4, gi));
+            p[0] = (ph.bulk + 1.0 / 3.0) * visc(ph, pr.T);
+            p[1] = pr.u;
+            p[2] = pr.v;
+            p[3] = pr.w;
+            p[4] = a.dQ[gi];
+            p[5] = a.dQ[2 * a.N + gi];
+            p[6] = a.dQ[4 * a.N + gi];
+            if (DIR == 1) p[7] = a.dQ[a.N + gi];
+        }
+        if (HYP && DIR != 0) {
+            p[NPQ] = a.L2S[gi];
+            p[NPQ + 1] = a.L4e[gi];
+        }
+    }
     // G index of each mixed input: gxx=0, gyx=1, gzx=2, gxy=3, gyy=4, gzy=5, gxz=6, gyz=7, gzz=8
     DEV static int gsrc(int m)
     {
@@ -414,10 +458,9 @@ struct PassMix {
     DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
     DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
     {
+        const double *p = v + NSLOT;
         if (NMIX > 0) {
-            const Phys &ph = a.ph;
-            const Prim pr = prim(ph, qf(a, 0, gi), qf(a, 1, gi), qf(a, 2, gi), qf(a, 3, gi), qf(a, 4, gi));
-            const double c3 = (ph.bulk + 1.0 / 3.0) * visc(ph, pr.T);
+            const double c3 = p[0];
             double fx, fy, fz;
             if (DIR == 1) {       // v: d/dy(dux/dx), d/dy(duy/dx), d/dy(duy/dz), d/dy(duz/dz)
                 fx = c3 * v[1];
@@ -428,14 +471,19 @@ struct PassMix {
                 fy = 0.0;
                 fz = c3 * v[0];
             }
-            a.dQ[gi] += fx;
-            if (DIR == 1) a.dQ[a.N + gi] += fy;
-            a.dQ[2 * a.N + gi] += fz;
-            a.dQ[4 * a.N + gi] += pr.u * fx + pr.v * fy + pr.w * fz;
+            a.dQ[gi] = p[4] + fx;
+            if (DIR == 1) a.dQ[a.N + gi] = p[7] + fy;
+            a.dQ[2 * a.N + gi] = p[5] + fz;
+            a.dQ[4 * a.N + gi] = p[6] + (p[1] * fx + p[2] * fy + p[3] * fz);
         }
         if (HYP) {
-            upd(&a.L2S[gi], v[NMIX], a.first);
-            upd(&a.L4e[gi], v[NMIX + 1], a.first);
+            if (DIR == 0) {
+                upd(&a.L2S[gi], v[NMIX], a.first);
+                upd(&a.L4e[gi], v[NMIX + 1], a.first);
+            } else {
+                a.L2S[gi] = a.first ? v[NMIX] : p[NPQ] + v[NMIX];
+                a.L4e[gi] = a.first ? v[NMIX + 1] : p[NPQ + 1] + v[NMIX + 1];
+            }
         }
     }
 };
@@ -525,6 +573,15 @@ struct PassHyperDiv {
     static constexpr int NSLOT = 4, NOUT = 0, NEARLY = 4;
     static constexpr bool REUSE = true;
     static constexpr int MIN_WAVES = 1;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 4;            // dQ 1, 2, 3, 5 before this pass
+    DEV static void prefetch(const Geo &, const PassArgs &a, long gi, double *p)
+    {
+        p[0] = a.dQ[gi];
+        p[1] = a.dQ[a.N + gi];
+        p[2] = a.dQ[2 * a.N + gi];
+        p[3] = a.dQ[4 * a.N + gi];
+    }
     DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
     {
         const long N = a.N;
@@ -572,10 +629,10 @@ struct PassHyperDiv {
     DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
     DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
     {
-        a.dQ[gi] += v[0];
-        a.dQ[a.N + gi] += v[1];
-        a.dQ[2 * a.N + gi] += v[2];
-        a.dQ[4 * a.N + gi] += v[3];
+        a.dQ[gi] = v[4] + v[0];
+        a.dQ[a.N + gi] = v[5] + v[1];
+        a.dQ[2 * a.N + gi] = v[6] + v[2];
+        a.dQ[4 * a.N + gi] = v[7] + v[3];
     }
 };
 
@@ -589,6 +646,10 @@ struct PassFilterQ {
     static constexpr int NSLOT = 5, NOUT = 0, NEARLY = 5;
     static constexpr bool REUSE = true;
     static constexpr int MIN_WAVES = 1;
+    // the RK update loads 15 fields per point: two points in flight at once
+    static constexpr int LOAD_UNROLL = KIND == 0 ? 2 : 8;
+    static constexpr int NPF = 0;
+    DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
     {
 #pragma unroll
@@ -667,12 +728,17 @@ __global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
         const double fX = tx * (2.0 * gxx + f1 * dil) + ty * t1 + tz * t2;
         const double fY = ty * (2.0 * gyy + f1 * dil) + tx * t1 + tz * t3;
         const double fZ = tz * (2.0 * gzz + f1 * dil) + tx * t2 + ty * t3;
-        a.dQ[gi] += fX;
-        a.dQ[N + gi] += fY;
-        a.dQ[2 * N + gi] += fZ;
-        a.dQ[4 * N + gi] += pr.u * fX + pr.v * fY + pr.w * fZ
-                            + mu * (2.0 * (gxx * gxx + gyy * gyy + gzz * gzz) + f1 * dil * dil)
-                            + mu * (t1 * t1 + t2 * t2 + t3 * t3);
+        // the y pass left its terms in dQ, the z and x passes theirs in dQp
+        const double *Pz = a.dQp, *Px = a.dQp + 5 * N;
+        a.dQ[gi] = ((a.dQ[gi] + Pz[gi]) + Px[gi]) + fX;
+        a.dQ[N + gi] = ((a.dQ[N + gi] + Pz[N + gi]) + Px[N + gi]) + fY;
+        a.dQ[2 * N + gi] = ((a.dQ[2 * N + gi] + Pz[2 * N + gi]) + Px[2 * N + gi]) + fZ;
+        a.dQ[3 * N + gi] = (a.dQ[3 * N + gi] + Pz[3 * N + gi]) + Px[3 * N + gi];
+        a.dQ[4 * N + gi] = ((a.dQ[4 * N + gi] + Pz[4 * N + gi]) + Px[4 * N + gi])
+                           + (pr.u * fX + pr.v * fY + pr.w * fZ
+                              + mu * (2.0 * (gxx * gxx + gyy * gyy + gzz * gzz) + f1 * dil * dil)
+                              + mu * (t1 * t1 + t2 * t2 + t3 * t3));
+        if (a.L2e) a.L2e[gi] = (a.L2e[gi] + a.L2ep[gi]) + a.L2ep[N + gi];
         if (hyp) {
             const double s12 = 0.5 * (gxy + gyx), s13 = 0.5 * (gxz + gzx), s23 = 0.5 * (gyz + gzy);
             a.S[gi] = sqrt(0.5 * (gxx * gxx + gyy * gyy + gzz * gzz + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
@@ -740,20 +806,26 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
 
 void pass_default_cfg(int kind, int *narrow, int *nt)
 {
-    *nt = 256;
+    // measured per pass at 256^3: 512-thread tiles where the pass is bound
+    // by LDS occupancy rather than by registers; M/2 points per lane
+    // ("narrow") where the pass holds many fields in registers
     switch (kind) {
-    case K_RKF_X: case K_MIXH_Z: case K_HDIV:
-        // measured at 256^3: these lose less to the tail of the grid
-        // with 512-thread tiles than they gain from a second resident block
-        *nt = 512;
+    case K_GRAD: case K_MIX_Y: case K_MIXH_Y:
         *narrow = 1;
+        *nt = 256;
         break;
-    case K_GRAD: case K_F_Y: case K_F_Z:
-    case K_MIX_Y: case K_MIX_Z: case K_MIXH_Y:
+    case K_MIX_Z: case K_MIXH_X: case K_MIXH_Z: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST:
+    case K_HDIV: case K_RKF_X: case K_F_Y: case K_F_Z:
         *narrow = 1;
+        *nt = 512;
+        break;
+    case K_LAP2_LAST:
+        *narrow = 0;
+        *nt = 512;
         break;
     default:
         *narrow = 0;
+        *nt = 256;
     }
 }
 
