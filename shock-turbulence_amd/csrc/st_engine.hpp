@@ -34,15 +34,6 @@ struct Geo {
     long npts;
 };
 
-// Global index of element s of line `line` in direction dir.
-DEV long gindex(const Geo &g, long line, int s)
-{
-    if (g.dir == 0) return line + (long)s * ((long)g.My * g.Mz);
-    if (g.dir == 1) return line * g.My + s;
-    const long j = line % g.My, i = line / g.My;
-    return j + (long)g.My * g.Mz * i + (long)s * g.My;
-}
-
 // Tile for this workgroup.  Workgroups are dispatched round-robin over the
 // 8 XCDs; consecutive tiles (which share 128-byte lines for the strided
 // directions) are handed to workgroups of the same XCD.
@@ -64,28 +55,62 @@ HD int lds_idx(int b, int k, int c, int P)
     return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
 }
 
-template <int M>
-HD int slot_size(int P, int B)
-{
-    return B * M * P;
-}
+// Memory-order walk of one tile by one thread.  The tile holds B = NT/P
+// lines of n = M*P points; the thread visits elements e = tid + NT*r,
+// r = 0..M-1.  x and z tiles are B adjacent lines (consecutive y, the
+// unit-stride direction), so a thread stays on one line b = tid mod B and
+// steps s = tid/B + r*P along it; a y tile is B*n contiguous points.
+template <int M, int P, int NT>
+struct Walk {
+    static constexpr int n = M * P, B = NT / P;
+    long base;            // global index of the thread's first element
+    long step;            // global index step per r
+    int b, s0;            // x and z: the thread's line and first point
 
-// Element e (0 .. B*n-1) of the tile in memory order -> (b, s).
-DEV void tile_elem(const Geo &g, int e, int &b, int &s)
-{
-    if (g.dir == 1) {
-        b = e / g.n;
-        s = e - b * g.n;
-    } else {
-        b = e & (g.B - 1);
-        s = e / g.B;
+    DEV Walk(const Geo &g, long tile)
+    {
+        const int t = threadIdx.x;
+        const long line0 = tile * B;
+        if (g.dir == 1) {
+            base = line0 * n + t;
+            step = NT;
+            b = 0;
+            s0 = 0;
+            return;
+        }
+        b = t & (B - 1);
+        s0 = t / B;
+        if (g.dir == 0) {
+            const long sx = (long)g.My * g.Mz;
+            base = line0 + b + s0 * sx;
+            step = P * sx;
+        } else {
+            const unsigned line = (unsigned)(line0 + b);
+            const unsigned i = line / (unsigned)g.My, j = line - i * (unsigned)g.My;
+            base = j + (long)g.My * g.Mz * i + (long)s0 * g.My;
+            step = (long)P * g.My;
+        }
     }
-}
+    // global index and (line, point) of the r-th element
+    DEV long at(const Geo &g, int r, int &bb, int &ss) const
+    {
+        if (g.dir == 1) {
+            const int e = threadIdx.x + NT * r;
+            bb = e / n;
+            ss = e & (n - 1);
+        } else {
+            bb = b;
+            ss = s0 + r * P;
+        }
+        return base + r * step;
+    }
+};
 
-// Per-lane view during the compute phase.
-template <int M>
+// Per-lane view during the compute phase; P lanes per line.
+template <int M, int P_>
 struct Lane {
-    int c, P, seg, b;      // chunk index, lanes per line, first lane of the line, line in tile
+    static constexpr int P = P_;
+    int c, seg, b;         // chunk index, first lane of the line, line in tile
     double *lds;
     int ss;                // slot size (doubles)
 
@@ -96,24 +121,23 @@ struct Lane {
 
 // ss = B*M*P = NT*M doubles: a compile-time constant for the kernel, so the
 // slot offsets fold into the LDS instructions' immediate offsets.
-template <int M>
-DEV Lane<M> make_lane(const Geo &g, double *lds, int ss)
+template <int M, int P>
+DEV Lane<M, P> make_lane(double *lds, int ss)
 {
-    Lane<M> L;
+    Lane<M, P> L;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    L.P = g.P;
-    L.c = lane & (g.P - 1);
+    L.c = lane & (P - 1);
     L.seg = lane - L.c;
-    L.b = wave * (64 / g.P) + lane / g.P;
+    L.b = wave * (64 / P) + lane / P;
     L.lds = lds;
     L.ss = ss;
     return L;
 }
 
 // Window of M own values plus HALO on each side, fetched from neighbours.
-template <int M>
-DEV void window(const double (&v)[M], const Lane<M> &L, double (&w)[M + 2 * HALO])
+template <int M, int P>
+DEV void window(const double (&v)[M], const Lane<M, P> &L, double (&w)[M + 2 * HALO])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) w[HALO + k] = v[k];
@@ -184,14 +208,16 @@ DEV void madd_to(const double *m, const double (&t)[K], const double (&x)[K], do
 // One in-row Kogge-Stone step at distance D: forward e_c += A^(M D) e_(c-D),
 // backward e_c += A^(M D) e_(c+D).  Rows are segment boundaries for P >= 16;
 // for P = 8 two lines share a row and the cross-line source is masked.
-template <int K, bool FWD, int D, int J>
-DEV void row_step(double (&e)[K], const OpTab &T, int c, int P)
+template <int K, bool FWD, int D, int J, int P>
+DEV void row_step(double (&e)[K], const OpTab &T, int c)
 {
     double t[K];
 #pragma unroll
     for (int i = 0; i < K; i++) {
         t[i] = dpp<FWD ? DPP_ROW_SHR + D : DPP_ROW_SHL + D>(e[i]);
-        if (P < 16 && (FWD ? c < D : c + D >= P)) t[i] = 0.0;
+        if constexpr (P < 16) {
+            if (FWD ? c < D : c + D >= P) t[i] = 0.0;
+        }
     }
     madd<K>(T.sc[J], t, e);
 }
@@ -206,13 +232,14 @@ DEV void sel(bool p, const double (&a)[K], const double (&b)[K], double (&y)[K])
 // Segmented scan of the chunk-end states e over the P lanes of each line.
 // On return e is the inclusive prefix (forward) or suffix (backward) state,
 // and tot the line's total (the state at its last, resp. first, lane).
-template <int K, bool FWD>
-DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c, int P)
+template <int K, bool FWD, int P>
+DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c)
 {
-    row_step<K, FWD, 1, 0>(e, T, c, P);
-    row_step<K, FWD, 2, 1>(e, T, c, P);
-    row_step<K, FWD, 4, 2>(e, T, c, P);
-    if (P == 8) {
+    static_assert(P == 8 || P == 16 || P == 32 || P == 64, "lanes per line");
+    row_step<K, FWD, 1, 0, P>(e, T, c);
+    row_step<K, FWD, 2, 1, P>(e, T, c);
+    row_step<K, FWD, 4, 2, P>(e, T, c);
+    if constexpr (P == 8) {
         // two lines per row: totals at row lanes 7 / 15 (forward), 0 / 8 (backward)
         const bool hi = (threadIdx.x & 8) != 0;
 #pragma unroll
@@ -221,76 +248,74 @@ DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c, int 
             const double b = FWD ? dpp<DPP_ROW_NEWBCAST + 7>(e[i]) : dpp<DPP_ROW_NEWBCAST + 0>(e[i]);
             tot[i] = hi ? a : b;
         }
-        return;
-    }
-    row_step<K, FWD, 8, 3>(e, T, c, P);
-    if (P == 16) {
-#pragma unroll
-        for (int i = 0; i < K; i++)
-            tot[i] = FWD ? dpp<DPP_ROW_NEWBCAST + 15>(e[i]) : dpp<DPP_ROW_NEWBCAST + 0>(e[i]);
-        return;
-    }
-    // P = 32 or 64: carry whole rows with uniform values read from one
-    // lane per row (its row-local end state), A16 = A^(16 M).
-    const double *A16 = T.sc[4];
-    const int row = (threadIdx.x >> 4) & 3;
-    double s0[K], s1[K], s2[K], s3[K], carry[K];
-#pragma unroll
-    for (int i = 0; i < K; i++) {
-        s0[i] = FWD ? rdlane<15>(e[i]) : rdlane<0>(e[i]);
-        s1[i] = FWD ? rdlane<31>(e[i]) : rdlane<16>(e[i]);
-        s2[i] = FWD ? rdlane<47>(e[i]) : rdlane<32>(e[i]);
-        s3[i] = FWD ? rdlane<63>(e[i]) : rdlane<48>(e[i]);
-    }
-    const double zero[K] = {};
-    if (P == 32) {
-        double t0[K], t1[K];
-        if (FWD) {
-            madd_to<K>(A16, s0, s1, t0);           // total of lanes 0..31
-            madd_to<K>(A16, s2, s3, t1);           // total of lanes 32..63
-            double c0[K];
-            sel<K>(row == 1, s0, zero, c0);
-            sel<K>(row == 3, s2, c0, carry);
-        } else {
-            madd_to<K>(A16, s1, s0, t0);
-            madd_to<K>(A16, s3, s2, t1);
-            double c0[K];
-            sel<K>(row == 0, s1, zero, c0);
-            sel<K>(row == 2, s3, c0, carry);
-        }
-        sel<K>(row >= 2, t1, t0, tot);
     } else {
-        double f1[K], f2[K], c0[K], c1[K];
-        if (FWD) {
-            madd_to<K>(A16, s0, s1, f1);           // prefix through lane 31
-            madd_to<K>(A16, f1, s2, f2);           // through lane 47
-            madd_to<K>(A16, f2, s3, tot);          // through lane 63
-            sel<K>(row == 1, s0, zero, c0);
-            sel<K>(row == 2, f1, c0, c1);
-            sel<K>(row == 3, f2, c1, carry);
+        row_step<K, FWD, 8, 3, P>(e, T, c);
+        if constexpr (P == 16) {
+#pragma unroll
+            for (int i = 0; i < K; i++)
+                tot[i] = FWD ? dpp<DPP_ROW_NEWBCAST + 15>(e[i]) : dpp<DPP_ROW_NEWBCAST + 0>(e[i]);
         } else {
-            madd_to<K>(A16, s3, s2, f2);           // suffix from lane 32
-            madd_to<K>(A16, f2, s1, f1);           // from lane 16
-            madd_to<K>(A16, f1, s0, tot);          // from lane 0
-            sel<K>(row == 0, f1, zero, c0);
-            sel<K>(row == 1, f2, c0, c1);
-            sel<K>(row == 2, s3, c1, carry);
+            // P = 32 or 64: carry whole rows with uniform values read from
+            // one lane per row (its row-local end state), A16 = A^(16 M).
+            const double *A16 = T.sc[4];
+            const int row = (threadIdx.x >> 4) & 3;
+            double s0[K], s1[K], s2[K], s3[K], carry[K];
+#pragma unroll
+            for (int i = 0; i < K; i++) {
+                s0[i] = FWD ? rdlane<15>(e[i]) : rdlane<0>(e[i]);
+                s1[i] = FWD ? rdlane<31>(e[i]) : rdlane<16>(e[i]);
+                s2[i] = FWD ? rdlane<47>(e[i]) : rdlane<32>(e[i]);
+                s3[i] = FWD ? rdlane<63>(e[i]) : rdlane<48>(e[i]);
+            }
+            const double zero[K] = {};
+            if constexpr (P == 32) {
+                double t0[K], t1[K], c0[K];
+                if (FWD) {
+                    madd_to<K>(A16, s0, s1, t0);           // total of lanes 0..31
+                    madd_to<K>(A16, s2, s3, t1);           // total of lanes 32..63
+                    sel<K>(row == 1, s0, zero, c0);
+                    sel<K>(row == 3, s2, c0, carry);
+                } else {
+                    madd_to<K>(A16, s1, s0, t0);
+                    madd_to<K>(A16, s3, s2, t1);
+                    sel<K>(row == 0, s1, zero, c0);
+                    sel<K>(row == 2, s3, c0, carry);
+                }
+                sel<K>(row >= 2, t1, t0, tot);
+            } else {
+                double f1[K], f2[K], c0[K], c1[K];
+                if (FWD) {
+                    madd_to<K>(A16, s0, s1, f1);           // prefix through lane 31
+                    madd_to<K>(A16, f1, s2, f2);           // through lane 47
+                    madd_to<K>(A16, f2, s3, tot);          // through lane 63
+                    sel<K>(row == 1, s0, zero, c0);
+                    sel<K>(row == 2, f1, c0, c1);
+                    sel<K>(row == 3, f2, c1, carry);
+                } else {
+                    madd_to<K>(A16, s3, s2, f2);           // suffix from lane 32
+                    madd_to<K>(A16, f2, s1, f1);           // from lane 16
+                    madd_to<K>(A16, f1, s0, tot);          // from lane 0
+                    sel<K>(row == 0, f1, zero, c0);
+                    sel<K>(row == 1, f2, c0, c1);
+                    sel<K>(row == 2, s3, c1, carry);
+                }
+            }
+            // the carry enters lane c over (c mod 16) + 1 chunks (forward)
+            // or 16 - (c mod 16) chunks (backward)
+            madd<K>(T.lp[FWD ? (c & 15) + 1 : 16 - (c & 15)], carry, e);
         }
     }
-    // the carry enters lane c over (c mod 16) + 1 chunks (forward) or
-    // 16 - (c mod 16) chunks (backward)
-    madd<K>(T.lp[FWD ? (c & 15) + 1 : 16 - (c & 15)], carry, e);
 }
 
 // Cyclic solve of one line per P lanes; r holds the lane's M right-hand
 // sides (the scale 1+rho^2 or 1/kappa already folded in) and returns x.
 // Order 1: x = (1 - rho S^-1)^-1 (1 - rho S)^-1 r,
 // order 2: x = (1 - a1 S^-1 - a2 S^-2)^-1 (1 - a1 S - a2 S^2)^-1 r.
-template <int K, int M>
-DEV void solve(double (&r)[M], const OpTab &T, const Lane<M> &L)
+template <int K, int M, int P>
+DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P> &L)
 {
     const double a1 = T.a1, a2 = T.a2;
-    const int P = L.P, c = L.c;
+    const int c = L.c;
     // forward recurrence inside the chunk (zero incoming state)
 #pragma unroll
     for (int k = 1; k < M; k++) {
@@ -300,7 +325,7 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M> &L)
     double e[K], tot[K], w[K], cin[K];
     e[0] = r[M - 1];
     if (K == 2) e[K - 1] = M > 1 ? r[M > 1 ? M - 2 : 0] : 0.0;
-    wave_scan<K, true>(e, tot, T, c, P);
+    wave_scan<K, true, P>(e, tot, T, c);
     // incoming state of chunk c: the exclusive prefix plus the cyclic total
     // carried around the line, A^(M c) (I - A^n)^-1 tot
 #pragma unroll
@@ -324,7 +349,7 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M> &L)
     }
     e[0] = r[0];
     if (K == 2) e[K - 1] = M > 1 ? r[M > 1 ? 1 : 0] : 0.0;
-    wave_scan<K, false>(e, tot, T, c, P);
+    wave_scan<K, false, P>(e, tot, T, c);
 #pragma unroll
     for (int i = 0; i < K; i++) {
         cin[i] = dpp<DPP_WAVE_SHL1>(e[i]);
@@ -343,9 +368,9 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M> &L)
 // ---- operators on a window -----------------------------------------------
 
 // First derivative: K = cadd*sx(1)*(N-1)*(1+rho^2).
-template <int M>
+template <int M, int P>
 DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M> &L, double (&out)[M])
+               const Lane<M, P> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -356,9 +381,9 @@ DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
 }
 
 // Second derivative: K = (N-1)^2*sx(2)*(1+rho^2).
-template <int M>
+template <int M, int P>
 DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M> &L, double (&out)[M])
+               const Lane<M, P> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -369,9 +394,9 @@ DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
 }
 
 // Eighth-order compact filter: K = 1/kappa.
-template <int M>
+template <int M, int P>
 DEV void op_filt(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-                 const Lane<M> &L, double (&out)[M])
+                 const Lane<M, P> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {

@@ -32,7 +32,7 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // (read-modify-write operands): prefetch() loads them during the load phase,
 // which visits the same points per thread as the store phase, and they
 // reach store_early() (REUSE) or store() appended after the results.
-template <int M, int NT, class PASS>
+template <int M, int P, int NT, class PASS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
 k_pass(Geo g, PassArgs a)
 {
@@ -45,42 +45,37 @@ k_pass(Geo g, PassArgs a)
     constexpr int NPF = PASS::NPF;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
-    const long tile = tile_of(g);
-    const long line0 = tile * g.B;
-    constexpr int ss = NT * M;           // slot_size: B*M*P with B*P = NT
+    const Walk<M, P, NT> W(g, tile_of(g));
+    constexpr int ss = NT * M;           // slot size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
 #pragma unroll(PASS::LOAD_UNROLL)
     for (int r = 0; r < M; r++) {
-        const int e = threadIdx.x + NT * r;
         int b, s;
-        tile_elem(g, e, b, s);
-        const long gi = gindex(g, line0 + b, s);
+        const long gi = W.at(g, r, b, s);
         double v[NS > 0 ? NS : 1];
         PASS::load(g, a, gi, v);
         if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, g.P);
+        const int li = lds_idx<M>(b, k, c, P);
 #pragma unroll
         for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
     }
     __syncthreads();
 
     // 2. compute (line mapping)
-    const Lane<M> L = make_lane<M>(g, lds, ss);
+    const Lane<M, P> L = make_lane<M, P>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
-    PASS::template compute<M>(g, a, L, out);
+    PASS::template compute<M, P>(g, a, L, out);
 
     // 3. store (through LDS, memory order)
     __syncthreads();
     if (RE) {
 #pragma unroll
         for (int r = 0; r < M; r++) {
-            const int e = threadIdx.x + NT * r;
             int b, s;
-            tile_elem(g, e, b, s);
-            const long gi = gindex(g, line0 + b, s);
-            const int li = lds_idx<M>(b, s % M, s / M, g.P);
+            const long gi = W.at(g, r, b, s);
+            const int li = lds_idx<M>(b, s % M, s / M, P);
             double v[NE + NPF > 0 ? NE + NPF : 1];
 #pragma unroll
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
@@ -99,12 +94,10 @@ k_pass(Geo g, PassArgs a)
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < M; r++) {
-        const int e = threadIdx.x + NT * r;
         int b, s;
-        tile_elem(g, e, b, s);
-        const long gi = gindex(g, line0 + b, s);
+        const long gi = W.at(g, r, b, s);
         const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, g.P);
+        const int li = lds_idx<M>(b, k, c, P);
         constexpr int NV = RE ? NO : NO + NE;
         double v[NV + NPF > 0 ? NV + NPF : 1];
 #pragma unroll
@@ -134,8 +127,8 @@ struct NoReuse {
 // ---------------------------------------------------------------------
 DEV double qf(const PassArgs &a, int comp, long gi) { return a.Q[comp * a.N + gi]; }
 
-template <int M>
-DEV void qown(const Lane<M> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
+template <int M, int P>
+DEV void qown(const Lane<M, P> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
 {
     q1 = L.ld(0, k);
     q2 = L.ld(1, k);
@@ -156,8 +149,8 @@ DEV void upd(double *p, double v, bool first)
 struct PassD1 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -174,8 +167,8 @@ struct PassD1 : NoReuse {
 struct PassD2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 1;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO], d2[M];
 #pragma unroll
@@ -200,8 +193,8 @@ struct PassD2 : NoReuse {
 struct PassFilt : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -215,8 +208,8 @@ struct PassFilt : NoReuse {
 struct PassGauss : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -244,10 +237,14 @@ struct PassGauss : NoReuse {
 // rho*E + P = rho T + rho |u|^2 / 2, and D2(T)/gamma replaces D2(T/gamma);
 // all three differ from the reference's evaluation only by rounding.
 // ---------------------------------------------------------------------
+// MW: occupancy target (waves per SIMD) handed to the register allocator;
+// 3 suits the unit-stride y lines, the strided x and z lines run faster with
+// the compiler's own choice.
+template <int MW>
 struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
-    static constexpr int MIN_WAVES = 3;   // 168 registers: 3 waves per SIMD
+    static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
@@ -274,8 +271,8 @@ struct PassGrad {
     // the Laplacian of e; then each velocity, whose slot takes its
     // derivative.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations.
-    template <int M>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M> &L, double (&out)[5][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -388,8 +385,7 @@ struct PassMix {
     static constexpr int NH = HYP ? 2 : 0;
     static constexpr int NSLOT = NMIX + NH, NOUT = 0, NEARLY = NMIX + NH;
     static constexpr bool REUSE = true;
-    // z passes run 512-thread tiles: 4 waves per SIMD keeps two resident
-    static constexpr int MIN_WAVES = DIR == 2 ? 4 : 1;
+    static constexpr int MIN_WAVES = 1;
     static constexpr int LOAD_UNROLL = 8;
     // prefetched for the store phase: (1/3 + bulk) mu, u, v, w and the
     // right-hand sides 1, 3, 5 (and 2 for y); L2S and L4e when they are
@@ -430,8 +426,8 @@ struct PassMix {
             v[NMIX + 1] = a.L2e[gi];
         }
     }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
     {
         // each result replaces its own input in LDS
 #pragma unroll 1
@@ -497,8 +493,8 @@ template <bool LAST>
 struct PassLap2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -535,8 +531,8 @@ struct PassGaussEta : NoReuse {
         v[0] = a.eta[gi];
         v[1] = a.eta[a.N + gi];
     }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&out)[2][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[2][M])
     {
 #pragma unroll
         for (int m = 0; m < 2; m++) {
@@ -612,8 +608,8 @@ struct PassHyperDiv {
         v[2] = tz;
         v[3] = uh * tx + vh * ty + wh * tz + kap * G[(9 + g.dir) * N + gi];
     }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
     {
 #pragma unroll 1
         for (int m = 0; m < 4; m++) {
@@ -664,8 +660,8 @@ struct PassFilterQ {
             }
         }
     }
-    template <int M>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M> &L, double (&)[1][M])
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
     {
         const OpTab &T = a.tab->op[2];
 #pragma unroll 1
@@ -753,30 +749,40 @@ __global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-template <int M, int NT, class PASS>
+template <int M, int P, int NT, class PASS>
 static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
 {
-    const size_t lds = (size_t)pass_slots<PASS>() * slot_size<M>(g.P, g.B) * sizeof(double);
+    const size_t lds = (size_t)pass_slots<PASS>() * NT * M * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static size_t lds_set = 0;
     if (lds > 64 * 1024 && lds > lds_set) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, NT, PASS>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, P, NT, PASS>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL((k_pass<M, NT, PASS>), dim3((unsigned)g.ntiles), dim3(NT), lds, s, g, a);
+    hipLaunchKernelGGL((k_pass<M, P, NT, PASS>), dim3((unsigned)g.ntiles), dim3(NT), lds, s, g, a);
     return hipGetLastError();
 }
 
+// The lane layouts (M points per lane, P lanes per line) that
+// choose_layout and its narrow variant (M/2, 2P) produce for lines of
+// 16..512 points.
 template <class PASS, int NT>
-static hipError_t launch_m(int M, const Geo &g, const PassArgs &a, hipStream_t s)
+static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStream_t s)
 {
-    switch (M) {
-    case 1: return launch_t<1, NT, PASS>(g, a, s);
-    case 2: return launch_t<2, NT, PASS>(g, a, s);
-    case 4: return launch_t<4, NT, PASS>(g, a, s);
-    case 8: return launch_t<8, NT, PASS>(g, a, s);
+    switch (M * 100 + P) {
+    case 116: return launch_t<1, 16, NT, PASS>(g, a, s);
+    case 208: return launch_t<2, 8, NT, PASS>(g, a, s);
+    case 216: return launch_t<2, 16, NT, PASS>(g, a, s);
+    case 408: return launch_t<4, 8, NT, PASS>(g, a, s);
+    case 416: return launch_t<4, 16, NT, PASS>(g, a, s);
+    case 432: return launch_t<4, 32, NT, PASS>(g, a, s);
+    case 464: return launch_t<4, 64, NT, PASS>(g, a, s);
+    case 808: return launch_t<8, 8, NT, PASS>(g, a, s);
+    case 816: return launch_t<8, 16, NT, PASS>(g, a, s);
+    case 832: return launch_t<8, 32, NT, PASS>(g, a, s);
+    case 864: return launch_t<8, 64, NT, PASS>(g, a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -799,8 +805,8 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
     const long lot = g.npts / g.n;
     if (lot % g.B) return hipErrorInvalidValue;
     g.ntiles = lot / g.B;
-    if (nt == 256) return launch_m<PASS, 256>(t.M, g, a, s);
-    if (nt == 512) return launch_m<PASS, 512>(t.M, g, a, s);
+    if (nt == 256) return launch_m<PASS, 256>(t.M, t.P, g, a, s);
+    if (nt == 512) return launch_m<PASS, 512>(t.M, t.P, g, a, s);
     return hipErrorInvalidValue;
 }
 
@@ -810,11 +816,11 @@ void pass_default_cfg(int kind, int *narrow, int *nt)
     // by LDS occupancy rather than by registers; M/2 points per lane
     // ("narrow") where the pass holds many fields in registers
     switch (kind) {
-    case K_GRAD: case K_MIX_Y: case K_MIXH_Y:
+    case K_GRAD: case K_MIX_Y: case K_MIX_Z: case K_MIXH_Y: case K_MIXH_Z:
         *narrow = 1;
         *nt = 256;
         break;
-    case K_MIX_Z: case K_MIXH_X: case K_MIXH_Z: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST:
+    case K_MIXH_X: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST:
     case K_HDIV: case K_RKF_X: case K_F_Y: case K_F_Z:
         *narrow = 1;
         *nt = 512;
@@ -836,7 +842,9 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
     case K_D2: return launch<PassD2>(t, nt, dir, My, Mz, Mx, a, s);
     case K_FILT: return launch<PassFilt>(t, nt, dir, My, Mz, Mx, a, s);
     case K_GAUSS: return launch<PassGauss>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_GRAD: return launch<PassGrad>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GRAD:
+        if (dir == 1) return launch<PassGrad<3>>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1>>(t, nt, dir, My, Mz, Mx, a, s);
     case K_MIX_Y: return launch<PassMix<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
     case K_MIX_Z: return launch<PassMix<2, false>>(t, nt, dir, My, Mz, Mx, a, s);
     case K_MIXH_X: return launch<PassMix<0, true>>(t, nt, dir, My, Mz, Mx, a, s);
