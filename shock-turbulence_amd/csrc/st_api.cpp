@@ -41,7 +41,7 @@ struct st_ctx {
     st::Phys ph;
     st::Coef C;
     st::DirTab th[6];               // [dir] M points per lane, [3+dir] M/2 points per lane
-    int cfg_narrow[NKIND], cfg_nt[NKIND];
+    int cfg_narrow[NKIND], cfg_nt[NKIND][3];
     st::DirTab *td = nullptr;
     double *buf = nullptr;
     double *Q, *QW, *dQ, *G, *L2e, *S, *Cb, *L2S, *L4e, *L4S, *eta, *Hc, *T5, *tB, *tC;
@@ -94,12 +94,12 @@ void prof_begin(st_ctx *c, int kind, Event &e)
     e.kind = kind;
     e.a = ev_get(c);
     e.b = ev_get(c);
-    hipEventRecord(e.a, c->s);
+    (void)hipEventRecord(e.a, c->s);
 }
 
 void prof_end(st_ctx *c, Event &e)
 {
-    hipEventRecord(e.b, c->s);
+    (void)hipEventRecord(e.b, c->s);
     c->ev.push_back(e);
 }
 
@@ -179,7 +179,11 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
     a.K2 = t.s2 * t.op[1].scale;
     Event e;
     if (c->prof) prof_begin(c, pk, e);
-    hipError_t err = st::run_pass(kind, t, c->cfg_nt[kind], dir, c->My, c->Mz, c->Mx, a, c->s);
+    // the widest configured tile that the line count and layout allow
+    int nt = c->cfg_nt[kind][dir];
+    const long lot = c->N / t.n;
+    while (nt > 256 && ((nt == 1024 && t.P < 32) || lot % (nt / t.P))) nt /= 2;
+    hipError_t err = st::run_pass(kind, t, nt, dir, c->My, c->Mz, c->Mx, a, c->s);
     if (c->prof) {
         prof_end(c, e);
         c->prof_fields[pk] = kind_fields(kind, a);
@@ -211,13 +215,16 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
     std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
     std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
-    // launch layouts; ST_NARROW / ST_NT512 (bit masks over pass kinds) override
+    // launch layouts; ST_NARROW / ST_NT512 / ST_NT1024 (bit masks over pass kinds) override
     {
-        const char *en = getenv("ST_NARROW"), *et = getenv("ST_NT512");
+        const char *en = getenv("ST_NARROW"), *et = getenv("ST_NT512"), *eu = getenv("ST_NT1024");
         for (int k = 0; k < NKIND; k++) {
-            st::pass_default_cfg(k, &c->cfg_narrow[k], &c->cfg_nt[k]);
+            for (int d = 0; d < 3; d++) {
+                st::pass_default_cfg(k, d, &c->cfg_narrow[k], &c->cfg_nt[k][d]);
+                if (et) c->cfg_nt[k][d] = ((strtoul(et, nullptr, 0) >> k) & 1) ? 512 : 256;
+                if (eu && ((strtoul(eu, nullptr, 0) >> k) & 1)) c->cfg_nt[k][d] = 1024;
+            }
             if (en) c->cfg_narrow[k] = (int)((strtoul(en, nullptr, 0) >> k) & 1);
-            if (et) c->cfg_nt[k] = ((strtoul(et, nullptr, 0) >> k) & 1) ? 512 : 256;
         }
     }
     const int n[3] = {c->Mx, c->My, c->Mz};
@@ -353,17 +360,17 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
 extern "C" void st_ctx_destroy(st_ctx *c)
 {
     if (!c) return;
-    if (c->s) hipStreamSynchronize(c->s);
+    if (c->s) (void)hipStreamSynchronize(c->s);
     for (auto &e : c->ev) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
-    for (auto e : c->pool) hipEventDestroy(e);
-    if (c->buf) hipFree(c->buf);
-    if (c->td) hipFree(c->td);
-    if (c->red) hipFree(c->red);
-    if (c->red_h) hipHostFree(c->red_h);
-    if (c->s) hipStreamDestroy(c->s);
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->buf) (void)hipFree(c->buf);
+    if (c->td) (void)hipFree(c->td);
+    if (c->red) (void)hipFree(c->red);
+    if (c->red_h) (void)hipHostFree(c->red_h);
+    if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
 }
 

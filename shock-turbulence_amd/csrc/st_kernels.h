@@ -42,8 +42,14 @@ enum PassKind {
 
 // Default launch layout of a pass kind: narrow = 1 uses the tables with half
 // the points per lane (twice the lanes per line); nt = threads per workgroup.
-void pass_default_cfg(int kind, int *narrow, int *nt);
+void pass_default_cfg(int kind, int dir, int *narrow, int *nt);
 hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+// per translation unit (st_pass_*.hip)
+hipError_t launch_ops(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s);
 
 }  // namespace st

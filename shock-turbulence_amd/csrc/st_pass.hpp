@@ -1,0 +1,218 @@
+// Line-pass kernel template and its launchers, shared by the pass
+// translation units (st_pass_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "st_engine.hpp"
+#include "st_kernels.h"
+
+namespace st {
+
+// ---------------------------------------------------------------------
+// generic pass kernel
+// ---------------------------------------------------------------------
+template <class PASS>
+constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PASS::NEARLY; }
+
+// Slots 0..NSLOT-1 hold the staged inputs; compute() may write final
+// results early (freeing their registers) into slots NSLOT..NSLOT+NEARLY-1,
+// or, with REUSE, over its own points of the input slots 0..NEARLY-1 once
+// it no longer reads them (each lane reads and writes only its own points
+// of the tile during compute).  The NOUT register results are staged into
+// slots 0..NOUT-1 after compute.  Without REUSE, store() receives per point
+// the NOUT late results followed by the NEARLY early ones; with REUSE the
+// early ones go first through store_early() and store() gets the late ones.
+// A pass may also name NPF values per point that its store phase reads
+// (read-modify-write operands): prefetch() loads them during the load phase,
+// which visits the same points per thread as the store phase, and they
+// reach store_early() (REUSE) or store() appended after the results.
+template <int M, int P, int NT, class PASS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
+k_pass(Geo g, PassArgs a)
+{
+    extern __shared__ double lds[];
+    constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
+    constexpr bool RE = PASS::REUSE;
+    static_assert(NO <= NS, "late outputs must fit in the input slots");
+    static_assert(NO > 0 || RE, "a pass without late outputs stores through store_early");
+    static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
+    constexpr int NPF = PASS::NPF;
+    static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
+    double pf[M][NPF > 0 ? NPF : 1];
+    const Walk<M, P, NT> W(g, tile_of(g));
+    constexpr int ss = NT * M;           // slot size: B*M*P with B*P = NT
+
+    // 1. load (memory order, coalesced)
+#pragma unroll PASS::LOAD_UNROLL
+    for (int r = 0; r < M; r++) {
+        int b, s;
+        const long gi = W.at(g, r, b, s);
+        double v[NS > 0 ? NS : 1];
+        PASS::load(g, a, gi, v);
+        if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
+        const int k = s % M, c = s / M;
+        const int li = lds_idx<M>(b, k, c, P);
+#pragma unroll
+        for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
+    }
+    __syncthreads();
+
+    // 2. compute (line mapping)
+    const Lane<M, P> L = make_lane<M, P>(lds, ss);
+    double out[NO > 0 ? NO : 1][M];
+    PASS::template compute<M, P>(g, a, L, out);
+
+    // 3. store (through LDS, memory order)
+    __syncthreads();
+    if (RE) {
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            int b, s;
+            const long gi = W.at(g, r, b, s);
+            const int li = lds_idx<M>(b, s % M, s / M, P);
+            double v[NE + NPF > 0 ? NE + NPF : 1];
+#pragma unroll
+            for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
+#pragma unroll
+            for (int j = 0; j < NPF; j++) v[NE + j] = pf[r][j];
+            PASS::store_early(g, a, gi, v);
+        }
+        if constexpr (NO == 0) return;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NO; j++) {
+#pragma unroll
+        for (int k = 0; k < M; k++) L.st(j, k, out[j][k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        int b, s;
+        const long gi = W.at(g, r, b, s);
+        const int k = s % M, c = s / M;
+        const int li = lds_idx<M>(b, k, c, P);
+        constexpr int NV = RE ? NO : NO + NE;
+        double v[NV + NPF > 0 ? NV + NPF : 1];
+#pragma unroll
+        for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
+        if (!RE) {
+#pragma unroll
+            for (int j = NO; j < NV; j++) v[j] = lds[(NS + j - NO) * ss + li];
+        }
+#pragma unroll
+        for (int j = 0; j < NPF; j++) v[NV + j] = pf[r][j];
+        PASS::store(g, a, gi, v);
+    }
+}
+
+// Passes whose early outputs have their own slots.
+struct NoReuse {
+    static constexpr bool REUSE = false;
+    static constexpr int MIN_WAVES = 1;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 0;
+    DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
+    DEV static void store_early(const Geo &, const PassArgs &, long, const double *) {}
+};
+
+// ---------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------
+DEV double qf(const PassArgs &a, int comp, long gi) { return a.Q[comp * a.N + gi]; }
+
+template <int M, int P>
+DEV void qown(const Lane<M, P> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
+{
+    q1 = L.ld(0, k);
+    q2 = L.ld(1, k);
+    q3 = L.ld(2, k);
+    q4 = L.ld(3, k);
+    q5 = L.ld(4, k);
+}
+
+DEV void upd(double *p, double v, bool first)
+{
+    *p = first ? v : *p + v;
+}
+
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+template <int M, int P, int NT, class PASS>
+static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
+{
+    const size_t lds = (size_t)pass_slots<PASS>() * NT * M * sizeof(double);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static size_t lds_set = 0;
+    if (lds > 64 * 1024 && lds > lds_set) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, P, NT, PASS>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL((k_pass<M, P, NT, PASS>), dim3((unsigned)g.ntiles), dim3(NT), lds, s, g, a);
+    return hipGetLastError();
+}
+
+// The lane layouts (M points per lane, P lanes per line) that
+// choose_layout and its narrow variant (M/2, 2P) produce for lines of
+// 16..512 points.
+template <class PASS, int NT>
+static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStream_t s)
+{
+    if constexpr (NT == 1024) {
+        // 16 or 32 lines per tile: lanes of 32 or 64 per line only
+        switch (M * 100 + P) {
+        case 432: return launch_t<4, 32, NT, PASS>(g, a, s);
+        case 464: return launch_t<4, 64, NT, PASS>(g, a, s);
+        case 832: return launch_t<8, 32, NT, PASS>(g, a, s);
+        case 864: return launch_t<8, 64, NT, PASS>(g, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (M * 100 + P) {
+        case 116: return launch_t<1, 16, NT, PASS>(g, a, s);
+        case 208: return launch_t<2, 8, NT, PASS>(g, a, s);
+        case 216: return launch_t<2, 16, NT, PASS>(g, a, s);
+        case 408: return launch_t<4, 8, NT, PASS>(g, a, s);
+        case 416: return launch_t<4, 16, NT, PASS>(g, a, s);
+        case 432: return launch_t<4, 32, NT, PASS>(g, a, s);
+        case 464: return launch_t<4, 64, NT, PASS>(g, a, s);
+        case 808: return launch_t<8, 8, NT, PASS>(g, a, s);
+        case 816: return launch_t<8, 16, NT, PASS>(g, a, s);
+        case 832: return launch_t<8, 32, NT, PASS>(g, a, s);
+        case 864: return launch_t<8, 64, NT, PASS>(g, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+}
+
+// t holds the tables of the chosen layout (M points per lane, P lanes per
+// line); nt threads per workgroup give B = nt/P lines per tile.
+template <class PASS>
+static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+{
+    Geo g;
+    g.dir = dir;
+    g.n = t.n;
+    g.My = My;
+    g.Mz = Mz;
+    g.Mx = Mx;
+    g.npts = (long)My * Mz * Mx;
+    g.P = t.P;
+    g.B = nt / t.P;
+    if (g.B < 1 || g.B * t.P != nt) return hipErrorInvalidValue;
+    const long lot = g.npts / g.n;
+    if (lot % g.B) return hipErrorInvalidValue;
+    g.ntiles = lot / g.B;
+    if (nt == 256) return launch_m<PASS, 256>(t.M, t.P, g, a, s);
+    if (nt == 512) return launch_m<PASS, 512>(t.M, t.P, g, a, s);
+    if (nt == 1024) return launch_m<PASS, 1024>(t.M, t.P, g, a, s);
+    return hipErrorInvalidValue;
+}
+
+
+}  // namespace st

@@ -1,0 +1,168 @@
+// First-derivative passes of the right-hand side.
+#include "st_pass.hpp"
+
+namespace st {
+
+// ---------------------------------------------------------------------
+// P1: first derivatives, own-direction second derivatives and convective
+// terms of one direction, from the state Q.
+//   RHS_x  src/rhsv_periodic_sf.f:36-461
+//   RHS_yz src/rhsv_periodic_sf.f:546-880 (y and z parts of the same terms)
+// The load phase stages the primitives (u, v, w, T, rho) of
+// src/rhsv_periodic_sf.f:92-104.  Per direction d (velocity u_d):
+//   dQ_i += D1(-rho u_i u_d/2 - delta_id P) - D1(rho u_d) u_i/2
+//           - (rho u_d/2) du_i/dx_d + w_id mu d2u_i/dx_d^2,  w_id = 4/3+bulk (i=d) else 1
+//   dQ_4 += -D1(rho u_d)
+//   dQ_5 += D1(-(rho T + rho |u|^2/2) u_d) + sum_i u_i w_id mu d2u_i/dx_d^2
+//           + Pr^-1 (mu d2T/dx_d^2 + mu' (dT/dx_d)^2)
+//   G(., d) = D1(u, v, w, T);  L2e += D2(T)/gamma   (e = T/gamma)
+// D1 of a sum replaces the reference's sum of D1s of the same operator,
+// rho*E + P = rho T + rho |u|^2 / 2, and D2(T)/gamma replaces D2(T/gamma);
+// all three differ from the reference's evaluation only by rounding.
+// ---------------------------------------------------------------------
+// MW: occupancy target (waves per SIMD) handed to the register allocator;
+// 3 suits the unit-stride y lines, the strided x and z lines run faster with
+// the compiler's own choice.
+template <int MW>
+struct PassGrad {
+    static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
+    static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = MW;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 0;
+    DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
+    {
+        const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
+        v[0] = pr.u;
+        v[1] = pr.v;
+        v[2] = pr.w;
+        v[3] = pr.T;
+        v[4] = a.Q[3 * a.N + gi];
+    }
+    // Accumulate into row i of out (one of the rows in MASK) with static
+    // register indices.
+    template <int MASK, int M>
+    DEV static void acc(double (&out)[5][M], int i, int k, double v)
+    {
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+            if ((MASK >> r) & 1) out[r][k] += (i == r) ? v : 0.0;
+    }
+    // Operator order: fluxes and the mass flux first, while u, v, w, T and
+    // rho are all needed; then T, whose slot takes dT/dx_d and rho's slot
+    // the Laplacian of e; then each velocity, whose slot takes its
+    // derivative.  The loop bodies are free of branches so that the
+    // accumulators stay in place across iterations.
+    template <int M, int P>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    {
+        const Phys &ph = a.ph;
+        const int d = g.dir;
+        const double pfac = ph.gm1 * ph.ginv;
+        double mu[M], rud[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            mu[k] = visc(ph, L.ld(3, k));
+            rud[k] = -0.5 * L.ld(4, k) * L.ld(d, k);
+#pragma unroll
+            for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+        }
+        // flux derivatives: momentum (with pressure) 0..2, energy 4
+#pragma unroll 1
+        for (int it = 0; it < 4; it++) {
+            const int comp = it == 3 ? 4 : it;
+            const int ci = it == 3 ? 0 : it;
+            const double pw = comp == d ? 1.0 : 0.0;
+            double f[M], w[M + 2 * HALO], df[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double u = L.ld(0, k), v = L.ld(1, k), ww = L.ld(2, k), T = L.ld(3, k), rho = L.ld(4, k);
+                const double ud = L.ld(d, k), ui = L.ld(ci, k);
+                const double fm = -0.5 * rho * ui * ud - pw * (rho * T * pfac);
+                const double fe = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
+                f[k] = it == 3 ? fe : fm;
+            }
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, df);
+#pragma unroll
+            for (int k = 0; k < M; k++) acc<0x17, M>(out, comp, k, df[k]);
+        }
+        // skew-symmetric part and continuity with the mass flux rho u_d
+        {
+            double f[M], w[M + 2 * HALO], sk[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(4, k) * L.ld(d, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, sk);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double t = -0.5 * sk[k];
+                out[0][k] += t * L.ld(0, k);
+                out[1][k] += t * L.ld(1, k);
+                out[2][k] += t * L.ld(2, k);
+                out[3][k] = -sk[k];
+            }
+        }
+        // T: first and second derivatives; the Laplacian of e = T/gamma is
+        // D2(T)/gamma (the operator is linear)
+        {
+            double f[M], w[M + 2 * HALO], gd[M], hd[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(3, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                L.st(3, k, gd[k]);
+                L.st(4, k, hd[k] * ph.ginv);
+                const double mup = dvisc(ph, f[k], mu[k]);
+                out[4][k] += ph.pr_inv * (mu[k] * hd[k] + mup * gd[k] * gd[k]);
+            }
+        }
+        // u, v, w: first and second derivatives
+#pragma unroll 1
+        for (int comp = 0; comp < 3; comp++) {
+            double f[M], w[M + 2 * HALO], gd[M], hd[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, gd);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, hd);
+            const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                L.st(comp, k, gd[k]);
+                const double vt = wd * mu[k] * hd[k];
+                acc<0x07, M>(out, comp, k, vt + rud[k] * gd[k]);
+                out[4][k] += f[k] * vt;
+            }
+        }
+    }
+    DEV static void store_early(const Geo &g, const PassArgs &a, long gi, const double *v)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.G[(g.dir * 3 + c) * a.N + gi] = v[c];
+        a.G[(9 + g.dir) * a.N + gi] = v[3];
+        if (a.L2e) upd(&a.L2e[gi], v[4], a.first);
+    }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    {
+#pragma unroll
+        for (int c = 0; c < 5; c++) upd(&a.dQ[c * a.N + gi], v[c], a.first);
+    }
+};
+
+
+hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+{
+    switch (kind) {
+    case K_GRAD:
+        if (dir == 1) return launch<PassGrad<3>>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1>>(t, nt, dir, My, Mz, Mx, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace st

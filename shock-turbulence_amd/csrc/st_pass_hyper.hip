@@ -1,0 +1,167 @@
+// Artificial-transport passes: second Laplacians, Gaussian smoothing, flux divergence.
+#include "st_pass.hpp"
+
+namespace st {
+
+// ---------------------------------------------------------------------
+// P3: Laplacian of Lap|S|.  The z pass (last) also forms the transport
+// drivers eta = (rho |Lap^2 S|, (rho/T) c |Lap^2 e|)
+// (src/hyper_dynamic.f:278-297).
+// ---------------------------------------------------------------------
+template <bool LAST>
+struct PassLap2 : NoReuse {
+    static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
+    {
+        double f[M], w[M + 2 * HALO];
+#pragma unroll
+        for (int k = 0; k < M; k++) f[k] = L.ld(0, k);
+        window<M>(f, L, w);
+        op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, out[0]);
+    }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    {
+        if (!LAST) {
+            upd(&a.L4S[gi], v[0], a.first);
+            return;
+        }
+        const Phys &ph = a.ph;
+        const double d4S = fabs(a.L4S[gi] + v[0]);
+        const double d4e = fabs(a.L4e[gi]);
+        const double q4 = qf(a, 3, gi);
+        const Prim pr = prim(ph, qf(a, 0, gi), qf(a, 1, gi), qf(a, 2, gi), q4, qf(a, 4, gi));
+        const double fac3 = (q4 / pr.T) * sqrt(ph.gamma * pr.p * pr.vs);
+        a.eta[gi] = q4 * d4S;
+        a.eta[a.N + gi] = fac3 * d4e;
+    }
+};
+
+// ---------------------------------------------------------------------
+// Gaussian smoothing of eta (in place).  The z pass (last) forms the
+// transport coefficients beta*, mu*, kappa* (src/hyper_dynamic.f:303-314).
+// ---------------------------------------------------------------------
+template <bool LAST>
+struct PassGaussEta : NoReuse {
+    static constexpr int NSLOT = 2, NOUT = 2, NEARLY = 0;
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[2])
+    {
+        v[0] = a.eta[gi];
+        v[1] = a.eta[a.N + gi];
+    }
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[2][M])
+    {
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            double f[M], w[M + 2 * HALO];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
+            window<M>(f, L, w);
+            op_gauss<M>(w, a.C, out[m]);
+        }
+    }
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
+    {
+        if (!LAST) {
+            a.eta[gi] = v[0];
+            a.eta[a.N + gi] = v[1];
+            return;
+        }
+        const Phys &ph = a.ph;
+        a.Hc[gi] = a.Cb[gi] * v[0] * ph.dx6;            // beta*
+        a.Hc[a.N + gi] = ph.cmu * v[0] * ph.dx6;        // mu*
+        a.Hc[2 * a.N + gi] = ph.ck * v[1] * ph.dx5;     // kappa*
+    }
+};
+
+// ---------------------------------------------------------------------
+// P5: divergence of the artificial stress and heat flux
+// (src/hyper_rhs.f:159-491).  For direction d the staged fields are the
+// momentum fluxes tau_xd, tau_yd, tau_zd and the energy flux
+// u tau_xd + v tau_yd + w tau_zd + kappa* dT/dx_d (the reference takes the
+// D1 of each energy term separately; the sum of the D1s is the D1 of the
+// sum).
+// ---------------------------------------------------------------------
+struct PassHyperDiv {
+    static constexpr int NSLOT = 4, NOUT = 0, NEARLY = 4;
+    static constexpr bool REUSE = true;
+    static constexpr int MIN_WAVES = 1;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = 4;            // dQ 1, 2, 3, 5 before this pass
+    DEV static void prefetch(const Geo &, const PassArgs &a, long gi, double *p)
+    {
+        p[0] = a.dQ[gi];
+        p[1] = a.dQ[a.N + gi];
+        p[2] = a.dQ[2 * a.N + gi];
+        p[3] = a.dQ[4 * a.N + gi];
+    }
+    DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
+    {
+        const long N = a.N;
+        const double *G = a.G;
+        const double beta = a.Hc[gi], mu = a.Hc[N + gi], kap = a.Hc[2 * N + gi];
+        const double gxx = G[gi], gyy = G[4 * N + gi], gzz = G[8 * N + gi];
+        const double dil = gxx + gyy + gzz;
+        const double q4 = a.Q[3 * N + gi];
+        const double uh = a.Q[gi] / q4, vh = a.Q[N + gi] / q4, wh = a.Q[2 * N + gi] / q4;
+        const double tt = 2.0 / 3.0;
+        const double bulkterm = (beta - tt * mu) * dil;
+        double tx, ty, tz;
+        if (g.dir == 0) {
+            tx = 2.0 * mu * gxx + bulkterm;
+            ty = mu * (G[3 * N + gi] + G[1 * N + gi]);
+            tz = mu * (G[6 * N + gi] + G[2 * N + gi]);
+        } else if (g.dir == 1) {
+            tx = mu * (G[3 * N + gi] + G[1 * N + gi]);
+            ty = 2.0 * mu * gyy + bulkterm;
+            tz = mu * (G[7 * N + gi] + G[5 * N + gi]);
+        } else {
+            tx = mu * (G[6 * N + gi] + G[2 * N + gi]);
+            ty = mu * (G[7 * N + gi] + G[5 * N + gi]);
+            tz = 2.0 * mu * gzz + bulkterm;
+        }
+        v[0] = tx;
+        v[1] = ty;
+        v[2] = tz;
+        v[3] = uh * tx + vh * ty + wh * tz + kap * G[(9 + g.dir) * N + gi];
+    }
+    template <int M, int P>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
+    {
+#pragma unroll 1
+        for (int m = 0; m < 4; m++) {
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
+        }
+    }
+    DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v)
+    {
+        a.dQ[gi] = v[4] + v[0];
+        a.dQ[a.N + gi] = v[5] + v[1];
+        a.dQ[2 * a.N + gi] = v[6] + v[2];
+        a.dQ[4 * a.N + gi] = v[7] + v[3];
+    }
+};
+
+
+hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+{
+    switch (kind) {
+    case K_LAP2: return launch<PassLap2<false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_LAP2_LAST: return launch<PassLap2<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA: return launch<PassGaussEta<false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_HDIV: return launch<PassHyperDiv>(t, nt, dir, My, Mz, Mx, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace st
