@@ -26,9 +26,9 @@ STAGES_PER_STEP = 11
 
 # kernel symbol of each pass (kind, direction) as rocprofv3 reports it
 KERNEL_SYMBOL = {
-    ("grad", "y"): "k_pass<4, 64, 256, PassGrad<3>>",
-    ("grad", "x"): "k_pass<4, 64, 1024, PassGrad<1>>",
-    ("grad", "z"): "k_pass<4, 64, 256, PassGrad<1>>",
+    ("grad", "y"): "k_pass<4, 64, 256, PassGrad<3, false>>",
+    ("grad", "x"): "k_pass<4, 64, 1024, PassGrad<1, true>>",
+    ("grad", "z"): "k_pass<4, 64, 256, PassGrad<1, false>>",
     "hyper_div": "k_pass<4, 64, 1024, PassHyperDiv>",
     "cross": "k_cross",
     "rk_filter_x": "k_pass<4, 64, 1024, PassFilterQ<0>>",

@@ -215,7 +215,8 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
     std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
     std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
-    // launch layouts; ST_NARROW / ST_NT512 / ST_NT1024 (bit masks over pass kinds) override
+    // launch layouts; ST_NARROW / ST_NT512 / ST_NT1024 (bit masks over pass
+    // kinds) and ST_NT_<kind> (per direction) override the defaults
     {
         const char *en = getenv("ST_NARROW"), *et = getenv("ST_NT512"), *eu = getenv("ST_NT1024");
         for (int k = 0; k < NKIND; k++) {
@@ -225,6 +226,15 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
                 if (eu && ((strtoul(eu, nullptr, 0) >> k) & 1)) c->cfg_nt[k][d] = 1024;
             }
             if (en) c->cfg_narrow[k] = (int)((strtoul(en, nullptr, 0) >> k) & 1);
+            // ST_NT_<k> = "nx,ny,nz": tile threads of pass kind k per direction
+            char var[32];
+            snprintf(var, sizeof(var), "ST_NT_%d", k);
+            if (const char *ev = getenv(var)) {
+                int v[3];
+                if (sscanf(ev, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+                    for (int d = 0; d < 3; d++)
+                        if (v[d] == 256 || v[d] == 512 || v[d] == 1024) c->cfg_nt[k][d] = v[d];
+            }
         }
     }
     const int n[3] = {c->Mx, c->My, c->Mz};

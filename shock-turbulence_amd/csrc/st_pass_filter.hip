@@ -14,7 +14,7 @@ struct PassFilterQ {
     static constexpr bool REUSE = true;
     static constexpr int MIN_WAVES = 1;
     // the RK update loads 15 fields per point: two points in flight at once
-    static constexpr int LOAD_UNROLL = KIND == 0 ? 2 : 8;
+    static constexpr int LOAD_UNROLL = KIND == 0 ? 1 : 8;
     static constexpr int NPF = 0;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])

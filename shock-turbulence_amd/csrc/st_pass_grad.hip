@@ -22,8 +22,9 @@ namespace st {
 // ---------------------------------------------------------------------
 // MW: occupancy target (waves per SIMD) handed to the register allocator;
 // 3 suits the unit-stride y lines, the strided x and z lines run faster with
-// the compiler's own choice.
-template <int MW>
+// the compiler's own choice.  LEAN selects the operator order that keeps
+// fewer values live (for 1024-thread tiles, which allow 128 registers).
+template <int MW, bool LEAN>
 struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
@@ -55,7 +56,7 @@ struct PassGrad {
     // derivative.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations.
     template <int M, int P>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    DEV static void compute_wide(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -140,6 +141,116 @@ struct PassGrad {
             }
         }
     }
+
+    // Operator order: fluxes and the mass flux first, while u, v, w, T and
+    // rho are all needed; then the velocities (u_d last), whose slots take
+    // their derivatives; then T, whose slot takes dT/dx_d and rho's slot
+    // the Laplacian of e.  The loop bodies are free of branches so that the
+    // accumulators stay in place across iterations, and each loop iteration
+    // keeps one window and one solve live at a time.
+    template <int M, int P>
+    DEV static void compute_lean(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    {
+        const Phys &ph = a.ph;
+        const int d = g.dir;
+        const double pfac = ph.gm1 * ph.ginv;
+        double mu[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            mu[k] = visc(ph, L.ld(3, k));
+#pragma unroll
+            for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+        }
+        // flux derivatives: momentum (with pressure) 0..2, energy 4
+#pragma unroll 1
+        for (int it = 0; it < 4; it++) {
+            const int comp = it == 3 ? 4 : it;
+            const int ci = it == 3 ? 0 : it;
+            const double pw = comp == d ? 1.0 : 0.0;
+            double f[M], w[M + 2 * HALO], df[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double u = L.ld(0, k), v = L.ld(1, k), ww = L.ld(2, k), T = L.ld(3, k), rho = L.ld(4, k);
+                const double ud = L.ld(d, k), ui = L.ld(ci, k);
+                const double fm = -0.5 * rho * ui * ud - pw * (rho * T * pfac);
+                const double fe = -((rho * T + 0.5 * rho * (u * u + v * v + ww * ww)) * ud);
+                f[k] = it == 3 ? fe : fm;
+            }
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, df);
+#pragma unroll
+            for (int k = 0; k < M; k++) acc<0x17, M>(out, comp, k, df[k]);
+        }
+        // skew-symmetric part and continuity with the mass flux rho u_d
+        {
+            double f[M], w[M + 2 * HALO], sk[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(4, k) * L.ld(d, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, sk);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double t = -0.5 * sk[k];
+                out[0][k] += t * L.ld(0, k);
+                out[1][k] += t * L.ld(1, k);
+                out[2][k] += t * L.ld(2, k);
+                out[3][k] = -sk[k];
+            }
+        }
+        // u, v, w (u_d last): second, then first derivatives
+#pragma unroll 1
+        for (int it = 0; it < 3; it++) {
+            const int comp = it == 2 ? d : (d + 1 + it) % 3;
+            const double wd = comp == d ? (4.0 / 3.0 + ph.bulk) : 1.0;
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(comp, k);
+            window<M>(f, L, w);
+            double vt[M];
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                vt[k] = wd * mu[k] * r[k];
+                out[4][k] += w[HALO + k] * vt[k];
+            }
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double rud = -0.5 * L.ld(4, k) * L.ld(d, k);
+                acc<0x07, M>(out, comp, k, vt[k] + rud * r[k]);
+                L.st(comp, k, r[k]);
+            }
+        }
+        // T: first and second derivatives; the Laplacian of e = T/gamma is
+        // D2(T)/gamma (the operator is linear)
+        {
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(3, k);
+            window<M>(f, L, w);
+            op_d2<M>(w, a.C, a.K2, a.tab->op[1], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                L.st(4, k, r[k] * ph.ginv);
+                out[4][k] += ph.pr_inv * mu[k] * r[k];
+            }
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                L.st(3, k, r[k]);
+                const double mup = dvisc(ph, w[HALO + k], mu[k]);
+                out[4][k] += ph.pr_inv * (mup * r[k] * r[k]);
+            }
+        }
+    }
+
+    template <int M, int P>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    {
+        if constexpr (LEAN) compute_lean<M, P>(g, a, L, out);
+        else compute_wide<M, P>(g, a, L, out);
+    }
+
     DEV static void store_early(const Geo &g, const PassArgs &a, long gi, const double *v)
     {
 #pragma unroll
@@ -159,8 +270,9 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
 {
     switch (kind) {
     case K_GRAD:
-        if (dir == 1) return launch<PassGrad<3>>(t, nt, dir, My, Mz, Mx, a, s);
-        return launch<PassGrad<1>>(t, nt, dir, My, Mz, Mx, a, s);
+        if (nt == 1024) return launch<PassGrad<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
+        if (dir == 1) return launch<PassGrad<3, false>>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }
