@@ -74,13 +74,16 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     case K_GRAD:
         *nt = dir == 0 ? 1024 : 256;
         break;
-    case K_MIX_Y: case K_MIX_Z: case K_MIXH_Y: case K_MIXH_Z:
+    case K_MIX_Y: case K_MIXH_Y: case K_F_Y:
         *nt = 256;
         break;
-    case K_GAUSS_ETA: case K_F_Y: case K_F_Z:
+    case K_MIX_Z: case K_MIXH_Z: case K_GAUSS_ETA: case K_F_Z:
         *nt = 512;
         break;
-    case K_MIXH_X: case K_GAUSS_ETA_LAST: case K_HDIV: case K_RKF_X:
+    case K_HDIV:
+        *nt = dir == 1 ? 256 : 1024;
+        break;
+    case K_MIXH_X: case K_GAUSS_ETA_LAST: case K_RKF_X:
         *nt = 1024;
         break;
     case K_LAP2: case K_LAP2_LAST:
