@@ -29,6 +29,15 @@ typedef struct {
     int nx, ny, nz;
     double period_x, period_y, period_z;
     double x_start, y_start, z_start;
+    /* 0: periodic in x (src/makefile_periodic:11-22).  1: the non-periodic-x
+     * link set (src/makefile_nonperiodic1d:10-17): x spans period_x with both
+     * end points (src/define_nonperiodic1d_sf.f:42), x derivatives and
+     * filters close with boundary rows (src/derivCDopt6_nonperiodic1d.f,
+     * src/filters_nonperiodic1d.f), the Gaussian mirrors at the ends, the
+     * artificial transport uses constant coefficients
+     * (src/hyper_routines.f:66-214) and the hyper dt limit applies from the
+     * first step (src/lddrk4_nonperiodic_sf.f:105). */
+    int nonperiodic_x;
 } st_grid;
 
 typedef struct {

@@ -26,6 +26,7 @@ class Params(ctypes.Structure):
         ("hyper_param", ctypes.c_int),
         ("cfl_cal", ctypes.c_double),
         ("d2_mode", ctypes.c_int), ("rk_scheme", ctypes.c_int), ("mutate_q", ctypes.c_int),
+        ("nonper_x", ctypes.c_int),
     ]
 
 
@@ -50,6 +51,8 @@ def _load():
     lib.or_filter_dir.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     lib.or_gauss_dir.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     lib.or_band_solve.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_long]
+    lib.or_np_lhs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    lib.or_np_band_solve.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_long]
     lib.or_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.or_compute_hyper.argtypes = [ctypes.c_void_p, _dp]
     lib.or_hyper_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -98,6 +101,16 @@ def band_solve(R, alpha, beta):
     return R
 
 
+def np_band_solve(rows, R):
+    """pentadiagonal_serial with band rows (n, 5) = (A, B, C, D, E) on each row of R."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).copy()
+    n = R.shape[-1]
+    assert rows.shape == (n, 5)
+    lib().or_np_band_solve(n, rows.ctypes.data_as(_dp), R.ctypes.data_as(_dp), R.size // n)
+    return R
+
+
 def rk_coeffs(scheme, loop):
     a = np.zeros(6)
     b = np.zeros(6)
@@ -110,10 +123,10 @@ class Oracle:
 
     def __init__(self, nx, ny, nz, period=(2 * np.pi,) * 3, Re=100.0, Pr=0.7, gamma=1.4,
                  vis_exp=0.76, bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5,
-                 d2_mode=0, rk_scheme=0, mutate_q=0, start=(0.0, 0.0, 0.0)):
+                 d2_mode=0, rk_scheme=0, mutate_q=0, start=(0.0, 0.0, 0.0), nonper_x=0):
         self.params = Params(nx, ny, nz, period[0], period[1], period[2], start[0], start[1], start[2],
                              Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal,
-                             d2_mode, rk_scheme, mutate_q)
+                             d2_mode, rk_scheme, mutate_q, nonper_x)
         self.shape = (ny, nz, nx)
         self.h = lib().or_create(ctypes.byref(self.params))
 
@@ -121,6 +134,12 @@ class Oracle:
         if getattr(self, "h", None):
             lib().or_destroy(self.h)
             self.h = None
+
+    def np_lhs(self, op):
+        """LHS band rows (n, 5) of the non-periodic x operator op (0 D1, 1 D2, 2 filter)."""
+        r = np.zeros((self.params.nx, 5))
+        lib().or_np_lhs(self.h, op, r.ctypes.data_as(_dp))
+        return r
 
     def metrics(self):
         out = np.zeros(9)

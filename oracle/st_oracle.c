@@ -254,6 +254,7 @@ struct or_ctx {
     double re_inv, pr_inv;
     double c2_d2, d2_d2;        /* D2 +-3 weight and centre weight */
     band_t *b1[3], *b2[3], *bf[3];
+    double *np_rows[3];         /* non-periodic x: LHS rows of D1, D2, filter (5*Mx each) */
     double *grad;               /* /GRAD/: dux_dx duy_dx duz_dx dux_dy duy_dy duz_dy
                                    dux_dz duy_dz duz_dz dTdx dTdy dTdz */
     double *hyp;                /* /HYPER/: beta_hyper, visc_hyper, k_hyper */
@@ -328,8 +329,9 @@ or_ctx *or_create(const or_params *p)
     const double start[3] = {p->x_start, p->y_start, p->z_start};
     for (int d = 0; d < 3; d++) {
         const int N = c->n[d];
-        /* src/define_periodic_sf.f:42-44 */
-        const double leng = period[d] * (double)(N - 1) / (double)N;
+        /* src/define_periodic_sf.f:42-44; a non-periodic x spans its
+           period with both end points (src/define_nonperiodic1d_sf.f:42) */
+        const double leng = (d == 0 && p->nonper_x) ? period[d] : period[d] * (double)(N - 1) / (double)N;
         /* src/grid.f:31-32 */
         c->sx1[d] = 1.0 / leng;
         c->sx2[d] = c->sx1[d] * c->sx1[d];
@@ -355,6 +357,11 @@ or_ctx *or_create(const or_params *p)
         c->c2_d2 = 0.0;
         c->d2_d2 = 2.0 * (a2_pade6 + b2_pade6);
     }
+    if (p->nonper_x)
+        for (int op = 0; op < 3; op++) {
+            c->np_rows[op] = malloc(sizeof(double) * 5 * c->n[0]);
+            or_np_lhs(c, op, c->np_rows[op]);
+        }
     c->grad = malloc(sizeof(double) * 12 * c->npts);
     c->hyp = malloc(sizeof(double) * 3 * c->npts);
     pset(c->grad, 0.0, (size_t)12 * c->npts);
@@ -368,6 +375,7 @@ or_ctx *or_create(const or_params *p)
 
 void or_destroy(or_ctx *c)
 {
+    for (int op = 0; op < 3; op++) free(c->np_rows[op]);
     if (!c) return;
     for (int d = 0; d < 3; d++) { band_free(c->b1[d]); band_free(c->b2[d]); band_free(c->bf[d]); }
     free(c->grad); free(c->hyp); free(c->arena); free(c);
@@ -382,8 +390,14 @@ void or_metrics(const or_ctx *c, double *out)
 /* ---------------------------------------------------------------- */
 /* DFDX/DFDY/DFDZ: src/derivCDopt6_periodic.f:164-277,400-512,631-744 */
 /* ---------------------------------------------------------------- */
+static void np_dfdx(or_ctx *c, const double *F, double *dF, int iadd, double cadd);
+static void np_d2fdx2(or_ctx *c, const double *F, double *dF, double *d2F, int iadd);
+static void np_filter_x(or_ctx *c, const double *F, double *F0);
+static void np_gauss_x(or_ctx *c, const double *F, double *Fg);
+
 void or_dfd(or_ctx *c, int d, const double *F, double *dF, int iadd, double cadd)
 {
+    if (d == 0 && c->p.nonper_x) { np_dfdx(c, F, dF, iadd, cadd); return; }
     const int n = dir_n(c, d);
     const long lot = dir_lot(c, d);
     /* fac = cadd * sx(1) * (Nx - 1)   (:181, :417, :648) */
@@ -418,6 +432,7 @@ void or_dfd(or_ctx *c, int d, const double *F, double *dF, int iadd, double cadd
    first derivative is produced by a DFDX(F,dF,0,1) call (:381). */
 void or_d2fd2(or_ctx *c, int d, const double *F, double *dF, double *d2F, int iadd)
 {
+    if (d == 0 && c->p.nonper_x) { np_d2fdx2(c, F, dF, d2F, iadd); return; }
     const int n = dir_n(c, d);
     const long lot = dir_lot(c, d);
     /* fac = ((Nx - 1) * (Nx - 1)) * sx(2)   (:298) */
@@ -454,6 +469,7 @@ void or_d2fd2(or_ctx *c, int d, const double *F, double *dF, double *d2F, int ia
 /* Compact_Filter_X/Y/Z :115-251, :254-390, :393-528 (F0 may alias F) */
 void or_filter_dir(or_ctx *c, int d, const double *F, double *F0)
 {
+    if (d == 0 && c->p.nonper_x) { np_filter_x(c, F, F0); return; }
     const int n = dir_n(c, d);
     const long lot = dir_lot(c, d);
     const band_t *b = c->bf[d];
@@ -496,6 +512,7 @@ void or_compact_filter(or_ctx *c, const double *F, double *F0, int nf)
 /* Gaussian_Filter_X/Y/Z :565-678, :680-791, :793-906 */
 void or_gauss_dir(or_ctx *c, int d, const double *F, double *Fg)
 {
+    if (d == 0 && c->p.nonper_x) { np_gauss_x(c, F, Fg); return; }
     const int n = dir_n(c, d);
     const long lot = dir_lot(c, d);
 #pragma omp parallel
@@ -514,6 +531,255 @@ void or_gauss_dir(or_ctx *c, int d, const double *F, double *Fg)
                                   + a2_filt_gauss * (f[i + 2] + f[i - 2])
                                   + a1_filt_gauss * (f[i + 1] + f[i - 1])
                                   + a0_filt_gauss * f[i];
+            }
+        }
+        free(fb);
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* non-periodic x operators                                          */
+/* ---------------------------------------------------------------- */
+/* closures: src/header:170-175 */
+static const double closure1 = 5.0 / 9.0, closure2 = 1.0 / 18.0;
+static const double closure3 = 1.0 / 36.0, closure4 = 7.0 / 9.0;
+static const double closure5 = 3.0 / 44.0, closure6 = 12.0 / 11.0;
+/* Visbal & Gaitonde near-boundary filter: src/LES_header:118-148 */
+static const double bndy_alpha_filt = 0.45;
+#define BA bndy_alpha_filt
+static const double bc1[5] = {15.0 / 16.0, 1.0 / 4.0, -3.0 / 8.0, 1.0 / 4.0, -1.0 / 16.0};
+static const double bc2[5] = {1.0 / 16.0 + (7.0 / 8.0) * BA, 3.0 / 4.0 + (1.0 / 2.0) * BA,
+                              3.0 / 8.0 + (1.0 / 4.0) * BA, -1.0 / 4.0 + (1.0 / 2.0) * BA,
+                              1.0 / 16.0 - (1.0 / 8.0) * BA};
+static const double bc3_0 = 5.0 / 8.0 + (3.0 / 4.0) * BA;
+static const double bc3_1 = (1.0 / 2.0 + BA) / 2.0;
+static const double bc3_2 = (-1.0 / 8.0 + BA / 4.0) / 2.0;
+#undef BA
+
+/* Rows 1..n of the LHS bands (A=lower-2, B=lower-1, C=diag, D=upper+1,
+   E=upper+2), stored as rows[5*i + (0..4)].
+   op 0: Setup_Derivatives first derivative, src/derivCDopt6_nonperiodic1d.f:41-158
+   op 1: second derivative :160-357
+   op 2: Init_Filters src/filters_nonperiodic1d.f:45-215 */
+int or_np_lhs(const or_ctx *c, int op, double *r)
+{
+    const int n = c->n[0];
+    double a, b;
+    if (op == 0) { a = 0.0; b = alpha1_pade6; }          /* beta1_pade6 = 0 (src/header:193-197) */
+    else if (op == 1) { a = 0.0; b = alpha2_pade6; }     /* beta2_pade6 = 0 (src/header:198-203) */
+    else { a = beta_filt4; b = alpha_filt4; }
+    for (int i = 0; i < n; i++) {
+        r[5 * i + 0] = a; r[5 * i + 1] = b; r[5 * i + 2] = 1.0; r[5 * i + 3] = b; r[5 * i + 4] = a;
+    }
+#define ROW(i, A, B, C, D, E) do { double *q = r + 5 * (i); q[0] = A; q[1] = B; q[2] = C; q[3] = D; q[4] = E; } while (0)
+    if (op == 0) {
+        ROW(0, 0.0, 0.0, 1.0, 2.0, 0.0);
+        ROW(1, 0.0, 1.0 / 6.0, 1.0, 1.0 / 2.0, 0.0);
+        ROW(2, 0.0, 1.0 / 3.0, 1.0, 1.0 / 3.0, 0.0);
+        ROW(n - 1, 0.0, 2.0, 1.0, 0.0, 0.0);
+        ROW(n - 2, 0.0, 1.0 / 2.0, 1.0, 1.0 / 6.0, 0.0);
+        ROW(n - 3, 0.0, 1.0 / 3.0, 1.0, 1.0 / 3.0, 0.0);
+    } else if (op == 1) {
+        ROW(0, 0.0, 0.0, 1.0, 11.0, 0.0);
+        ROW(1, 0.0, 1.0e-1, 1.0, -7.0 / 20.0, 0.0);
+        ROW(2, 0.0, 2.0 / 11.0, 1.0, 2.0 / 11.0, 0.0);
+        ROW(n - 1, 0.0, 11.0, 1.0, 0.0, 0.0);
+        ROW(n - 2, 0.0, -7.0 / 20.0, 1.0, 1.0e-1, 0.0);
+        ROW(n - 3, 0.0, 2.0 / 11.0, 1.0, 2.0 / 11.0, 0.0);
+    } else {
+        ROW(0, 0.0, 0.0, 1.0, 0.0, 0.0);
+        ROW(1, 0.0, bndy_alpha_filt, 1.0, bndy_alpha_filt, 0.0);
+        ROW(2, 0.0, bndy_alpha_filt, 1.0, bndy_alpha_filt, 0.0);
+        ROW(3, beta_filt4, alpha_filt4, 1.0, alpha_filt4, beta_filt4);
+        ROW(n - 1, 0.0, 0.0, 1.0, 0.0, 0.0);
+        ROW(n - 2, 0.0, bndy_alpha_filt, 1.0, bndy_alpha_filt, 0.0);
+        ROW(n - 3, 0.0, bndy_alpha_filt, 1.0, bndy_alpha_filt, 0.0);
+        ROW(n - 4, beta_filt4, alpha_filt4, 1.0, alpha_filt4, beta_filt4);
+    }
+#undef ROW
+    return 0;
+}
+
+/* pentadiagonal_serial, src/pentadiagonal.f90:319-374, for one line:
+   the bands are copied per call (the reference copies them, :42) and
+   tiny(1.0) (single precision) guards each pivot. */
+static void np_solve_line(int n, const double *rows, double *R, double *w)
+{
+    double *A = w, *B = w + n, *C = w + 2 * n, *D = w + 3 * n, *E = w + 4 * n;
+    const double tiny1 = 1.17549435082228750797e-38;    /* tiny(1.0) */
+    for (int i = 0; i < n; i++) {
+        A[i] = rows[5 * i]; B[i] = rows[5 * i + 1]; C[i] = rows[5 * i + 2];
+        D[i] = rows[5 * i + 3]; E[i] = rows[5 * i + 4];
+    }
+    /* 0-based: reference row i+1 is index i */
+    for (int i = 0; i < n - 2; i++) {
+        double cst = B[i + 1] / (C[i] + tiny1);
+        C[i + 1] = C[i + 1] - D[i] * cst;
+        D[i + 1] = D[i + 1] - E[i] * cst;
+        R[i + 1] = R[i + 1] - R[i] * cst;
+        cst = A[i + 2] / (C[i] + tiny1);
+        B[i + 2] = B[i + 2] - D[i] * cst;
+        C[i + 2] = C[i + 2] - E[i] * cst;
+        R[i + 2] = R[i + 2] - R[i] * cst;
+    }
+    {
+        const double cst = B[n - 1] / (C[n - 2] + tiny1);
+        C[n - 1] = C[n - 1] - D[n - 2] * cst;
+        R[n - 1] = R[n - 1] - R[n - 2] * cst;
+    }
+    R[n - 1] = R[n - 1] / (C[n - 1] + tiny1);
+    R[n - 2] = (R[n - 2] - D[n - 2] * R[n - 1]) / (C[n - 2] + tiny1);
+    for (int i = n - 3; i >= 0; i--)
+        R[i] = (R[i] - D[i] * R[i + 1] - E[i] * R[i + 2]) / (C[i] + tiny1);
+}
+
+void or_np_band_solve(int n, const double *rows, double *R, long nlines)
+{
+#pragma omp parallel
+    {
+        double *w = malloc(sizeof(double) * 5 * n);
+#pragma omp for schedule(static)
+        for (long l = 0; l < nlines; l++) np_solve_line(n, rows, R + (size_t)l * n, w);
+        free(w);
+    }
+}
+
+/* DFDX src/derivCDopt6_nonperiodic1d.f:393-541 (single rank: both end
+   closures, :449-506) */
+static void np_dfdx(or_ctx *c, const double *F, double *dF, int iadd, double cadd)
+{
+    const int n = c->n[0];
+    const long lot = dir_lot(c, 0);
+    const double fac = cadd * c->sx1[0] * (double)(n - 1);
+#pragma omp parallel
+    {
+        double *f = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * 5 * n);
+#pragma omp for schedule(static)
+        for (long l = 0; l < lot; l++) {
+            long base, st;
+            dir_line(c, 0, l, &base, &st);
+            for (int i = 0; i < n; i++) f[i] = F[base + i * st];
+            for (int i = 3; i < n - 3; i++)
+                W[i] = fac * (c1_pade6 * (f[i + 3] - f[i - 3]) + b1_pade6 * (f[i + 2] - f[i - 2])
+                              + a1_pade6 * (f[i + 1] - f[i - 1]));
+            W[2] = fac * (closure3 * (f[4] - f[0]) + closure4 * (f[3] - f[1]));
+            W[1] = fac * (-closure1 * f[0] - 5.0e-1 * f[1] + f[2] + closure2 * f[3]);
+            W[0] = fac * (-2.5 * f[0] + 2.0 * f[1] + 5.0e-1 * f[2]);
+            W[n - 3] = fac * (closure3 * (f[n - 1] - f[n - 5]) + closure4 * (f[n - 2] - f[n - 4]));
+            W[n - 2] = fac * (closure1 * f[n - 1] + 5.0e-1 * f[n - 2] - f[n - 3] - closure2 * f[n - 4]);
+            W[n - 1] = fac * (2.5 * f[n - 1] - 2.0 * f[n - 2] - 5.0e-1 * f[n - 3]);
+            np_solve_line(n, c->np_rows[0], W, w);
+            if (iadd == 0)
+                for (int i = 0; i < n; i++) dF[base + i * st] = W[i];
+            else
+                for (int i = 0; i < n; i++) dF[base + i * st] = dF[base + i * st] + W[i];
+        }
+        free(f); free(W); free(w);
+    }
+}
+
+/* D2FDX2 :544-705; interior with the D2 mapping of the periodic routines
+   (c2 on +-3, d2 at the centre) */
+static void np_d2fdx2(or_ctx *c, const double *F, double *dF, double *d2F, int iadd)
+{
+    const int n = c->n[0];
+    const long lot = dir_lot(c, 0);
+    const double fac = (double)((n - 1) * (n - 1)) * c->sx2[0];
+    const double c2 = c->c2_d2, dd = c->d2_d2;
+#pragma omp parallel
+    {
+        double *f = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * 5 * n);
+#pragma omp for schedule(static)
+        for (long l = 0; l < lot; l++) {
+            long base, st;
+            dir_line(c, 0, l, &base, &st);
+            for (int i = 0; i < n; i++) f[i] = F[base + i * st];
+            for (int i = 3; i < n - 3; i++)
+                W[i] = fac * (c2 * (f[i + 3] + f[i - 3]) + b2_pade6 * (f[i + 2] + f[i - 2])
+                              + a2_pade6 * (f[i + 1] + f[i - 1]) - dd * f[i]);
+            W[2] = fac * (closure5 * (f[0] - 2.0 * f[2] + f[4]) + closure6 * (f[1] - 2.0 * f[2] + f[3]));
+            W[1] = fac * (1.2375 * f[0] - 3.0 * f[1] + 2.325 * f[2] - 6.0e-1 * f[3] + 3.75e-2 * f[4]);
+            W[0] = fac * (1.3e1 * f[0] - 2.7e1 * f[1] + 1.5e1 * f[2] - f[3]);
+            W[n - 3] = fac * (closure5 * (f[n - 1] - 2.0 * f[n - 3] + f[n - 5])
+                              + closure6 * (f[n - 2] - 2.0 * f[n - 3] + f[n - 4]));
+            W[n - 2] = fac * (1.2375 * f[n - 1] - 3.0 * f[n - 2] + 2.325 * f[n - 3] - 6.0e-1 * f[n - 4]
+                              + 3.75e-2 * f[n - 5]);
+            W[n - 1] = fac * (1.3e1 * f[n - 1] - 2.7e1 * f[n - 2] + 1.5e1 * f[n - 3] - f[n - 4]);
+            np_solve_line(n, c->np_rows[1], W, w);
+            for (int i = 0; i < n; i++) d2F[base + i * st] = W[i];
+        }
+        free(f); free(W); free(w);
+    }
+    if (iadd == 1) np_dfdx(c, F, dF, 0, 1.0);          /* :697 */
+}
+
+/* Compact_Filter_X src/filters_nonperiodic1d.f:257-456 */
+static void np_filter_x(or_ctx *c, const double *F, double *F0)
+{
+    const int n = c->n[0];
+    const long lot = dir_lot(c, 0);
+#pragma omp parallel
+    {
+        double *f = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * 5 * n);
+#pragma omp for schedule(static)
+        for (long l = 0; l < lot; l++) {
+            long base, st;
+            dir_line(c, 0, l, &base, &st);
+            for (int i = 0; i < n; i++) f[i] = F[base + i * st];
+            for (int i = 4; i < n - 4; i++)
+                W[i] = a4_filt4 * (f[i + 4] + f[i - 4]) + a3_filt4 * (f[i + 3] + f[i - 3])
+                     + a2_filt4 * (f[i + 2] + f[i - 2]) + a1_filt4 * (f[i + 1] + f[i - 1]) + a0_filt4 * f[i];
+            W[0] = bc1[0] * f[0] + bc1[1] * f[1] + bc1[2] * f[2] + bc1[3] * f[3] + bc1[4] * f[4];
+            W[1] = bc2[0] * f[0] + bc2[1] * f[1] + bc2[2] * f[2] + bc2[3] * f[3] + bc2[4] * f[4];
+            W[2] = bc3_2 * f[4] + bc3_1 * f[3] + bc3_0 * f[2] + bc3_1 * f[1] + bc3_2 * f[0];
+            {
+                const double a0 = 11.0 / 16.0 + (5.0 / 8.0) * alpha_filt4 - (5.0 / 8.0) * beta_filt4;
+                const double a1 = (15.0 / 32.0 + (17.0 / 16.0) * alpha_filt4 + (15.0 / 16.0) * beta_filt4) / 2.0;
+                const double a2 = (-3.0 / 16.0 + (3.0 / 8.0) * alpha_filt4 + (13.0 / 8.0) * beta_filt4) / 2.0;
+                const double a3 = (1.0 / 32.0 - alpha_filt4 / 16.0 + beta_filt4 / 16.0) / 2.0;
+                W[3] = a3 * f[6] + a2 * f[5] + a1 * f[4] + a0 * f[3] + a1 * f[2] + a2 * f[1] + a3 * f[0];
+                W[n - 4] = a3 * f[n - 7] + a2 * f[n - 6] + a1 * f[n - 5] + a0 * f[n - 4] + a1 * f[n - 3]
+                         + a2 * f[n - 2] + a3 * f[n - 1];
+            }
+            W[n - 1] = bc1[0] * f[n - 1] + bc1[1] * f[n - 2] + bc1[2] * f[n - 3] + bc1[3] * f[n - 4] + bc1[4] * f[n - 5];
+            W[n - 2] = bc2[0] * f[n - 1] + bc2[1] * f[n - 2] + bc2[2] * f[n - 3] + bc2[3] * f[n - 4] + bc2[4] * f[n - 5];
+            W[n - 3] = bc3_2 * f[n - 5] + bc3_1 * f[n - 4] + bc3_0 * f[n - 3] + bc3_1 * f[n - 2] + bc3_2 * f[n - 1];
+            np_solve_line(n, c->np_rows[2], W, w);
+            for (int i = 0; i < n; i++) F0[base + i * st] = W[i];
+        }
+        free(f); free(W); free(w);
+    }
+}
+
+/* Gaussian_Filter_X src/filters_nonperiodic1d.f:769-943: symmetry about
+   the end points, f(1-m) = f(1+m) and f(N+m) = f(N-m) */
+static void np_gauss_x(or_ctx *c, const double *F, double *Fg)
+{
+    const int n = c->n[0];
+    const long lot = dir_lot(c, 0);
+#pragma omp parallel
+    {
+        double *fb = malloc(sizeof(double) * (n + 8));
+        double *f = fb + 4;
+#pragma omp for schedule(static)
+        for (long l = 0; l < lot; l++) {
+            long base, st;
+            dir_line(c, 0, l, &base, &st);
+            for (int i = 0; i < n; i++) f[i] = F[base + i * st];
+            for (int m = 1; m <= 4; m++) { f[-m] = f[m]; f[n - 1 + m] = f[n - 1 - m]; }
+            for (int i = 0; i < n; i++) {
+                double g;
+                if (i == 0 || i == n - 1) {
+                    const int s = i == 0 ? 1 : -1;
+                    g = a4_filt_gauss * (2.0 * f[i + 4 * s]) + a3_filt_gauss * (2.0 * f[i + 3 * s])
+                      + a2_filt_gauss * (2.0 * f[i + 2 * s]) + a1_filt_gauss * (2.0 * f[i + s])
+                      + a0_filt_gauss * f[i];
+                } else {
+                    g = a4_filt_gauss * (f[i + 4] + f[i - 4]) + a3_filt_gauss * (f[i + 3] + f[i - 3])
+                      + a2_filt_gauss * (f[i + 2] + f[i - 2]) + a1_filt_gauss * (f[i + 1] + f[i - 1])
+                      + a0_filt_gauss * f[i];
+                }
+                Fg[base + i * st] = g;
             }
         }
         free(fb);
@@ -908,8 +1174,12 @@ void or_compute_hyper(or_ctx *c, const double *Q)
     double *tmp2 = ws_get(c, 2);
     double *eta = ws_get(c, 3), *etag = ws_get(c, 3);
     const double C_mu4 = 2.0e-3, C_k4 = 1.0e-2;              /* :162-163 */
+    /* The non-periodic-x link set uses src/hyper_routines.f:66-214, the
+       same chain with constant coefficients (C_beta4 = 1, :99-101) and two
+       smoothed fields (:190-198). */
+    const int stat = c->p.nonper_x;
 
-    compute_coeff1(c, Q, Cb);                                /* :214 */
+    if (!stat) compute_coeff1(c, Q, Cb);                     /* :214 */
     /* :220-222 U = Q/rho is not used afterwards; omitted. */
     /* Sij (:227-245) and contraction (:269, Compute_Contraction :12-33) */
     PFOR {
@@ -939,11 +1209,11 @@ void or_compute_hyper(or_ctx *c, const double *Q)
         eta[2 * N + l] = Q4[l] * a4S;
         eta[N + l] = fac3 * a4e;
     }
-    or_gaussian_filter(c, eta, etag, 3);                     /* :300 */
+    or_gaussian_filter(c, eta, etag, stat ? 2 : 3);          /* :300; static :198 */
     const double DX = c->D[0];
     double *beta = c->hyp, *visc = c->hyp + N, *kh = c->hyp + 2 * N;
-    PFOR {                                                   /* :303-314 */
-        beta[l] = Cb[l] * etag[2 * N + l] * pow(DX, 6.0);
+    PFOR {                                                   /* :303-314; static :204-206 */
+        beta[l] = stat ? 1.0 * etag[l] * pow(DX, 6.0) : Cb[l] * etag[2 * N + l] * pow(DX, 6.0);
         visc[l] = C_mu4 * etag[l] * pow(DX, 6.0);
         kh[l] = C_k4 * etag[N + l] * pow(DX, 5.0);
     }
@@ -1076,7 +1346,9 @@ double or_compute_tstep(or_ctx *c, const double *Q, double cfl_resv, int step)
 {
     const long N = c->npts;
     double Tstep = c->p.cfl_cal / cfl_resv;
-    if (c->p.hyper_param == 1 && step > 1) {
+    /* src/lddrk4_nonperiodic_sf.f:105 drops the step > 1 guard; before the
+       first Compute_hyper the coefficients are zero and the limits infinite */
+    if (c->p.hyper_param == 1 && (step > 1 || c->p.nonper_x)) {
         const double gamma = c->p.gamma;
         const double del = c->D[0];
         const double fac = gamma * (gamma - 1.0);

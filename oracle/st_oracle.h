@@ -32,6 +32,8 @@ typedef struct {
     int d2_mode;                     /* 0: consistent FDL3DI D2, 1: as shipped */
     int rk_scheme;                   /* 0: CC5 then LDDRK6; 1: LDDRK5 then LDDRK6; 2: CC5 only */
     int mutate_q;                    /* 1: replay RHS_yz's in-place momentum round trip */
+    int nonper_x;                    /* 1: x non-periodic (src/derivCDopt6_nonperiodic1d.f,
+                                        src/filters_nonperiodic1d.f); y, z periodic */
 } or_params;
 
 typedef struct or_ctx or_ctx;
@@ -51,6 +53,12 @@ void or_compact_filter(or_ctx *c, const double *F, double *F0, int nf);
 void or_gaussian_filter(or_ctx *c, const double *F, double *Fg, int nf);
 void or_filter_dir(or_ctx *c, int dir, const double *F, double *F0);
 void or_gauss_dir(or_ctx *c, int dir, const double *F, double *Fg);
+/* LHS band rows (A..E = lower-2 .. upper+2, 5*n doubles) of the non-periodic
+   x operators: op 0 first derivative, 1 second derivative, 2 compact filter */
+int or_np_lhs(const or_ctx *c, int op, double *rows5n);
+/* pentadiagonal_serial (src/pentadiagonal.f90:319-374) on nlines lines of
+   length n (line-major R) with the given band rows */
+void or_np_band_solve(int n, const double *rows5n, double *R, long nlines);
 
 /* cyclic band solve with constant bands (beta, alpha, 1, alpha, beta),
  * lines of length n with element stride 1, nlines consecutive lines

@@ -131,7 +131,7 @@ st::PassArgs base_args(st_ctx *c)
     a.G = c->G;
     a.L2e = c->p.hyper_param ? c->L2e : nullptr;
     a.S = c->S;
-    a.Cb = c->Cb;
+    a.Cb = c->g.nonperiodic_x ? nullptr : c->Cb;
     a.L2S = c->L2S;
     a.L4e = c->L4e;
     a.L4S = c->L4S;
@@ -198,7 +198,12 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
 // ---------------------------------------------------------------------
 // creation
 // ---------------------------------------------------------------------
-static double leng_of(double period, int n) { return period * (double)(n - 1) / (double)n; }
+// src/define_periodic_sf.f:42-44; a non-periodic direction spans its period
+// with both end points (src/define_nonperiodic1d_sf.f:42)
+static double leng_of(double period, int n, int nonper = 0)
+{
+    return nonper ? period : period * (double)(n - 1) / (double)n;
+}
 
 extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out)
 {
@@ -237,6 +242,8 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
             }
         }
     }
+    if (grid->nonperiodic_x && !getenv("ST_NT1024"))
+        for (int k = 0; k < NKIND; k++) c->cfg_nt[k][0] = 256;   // boundary tables cost registers
     const int n[3] = {c->Mx, c->My, c->Mz};
     const double period[3] = {grid->period_x, grid->period_y, grid->period_z};
     const double start[3] = {grid->x_start, grid->y_start, grid->z_start};
@@ -270,8 +277,9 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     c->ph.cmu = 2.0e-3;
     c->ph.ck = 1.0e-2;
     c->ph.cfl_cal = params->cfl_cal;
+    const int np[3] = {grid->nonperiodic_x ? 1 : 0, 0, 0};
     for (int d = 0; d < 3; d++) {
-        const double leng = leng_of(period[d], n[d]);
+        const double leng = leng_of(period[d], n[d], np[d]);
         const double g1 = start[d] + leng * 0.0 / (double)(n[d] - 1);
         const double g2 = start[d] + leng * 1.0 / (double)(n[d] - 1);
         c->ph.D[d] = g2 - g1;                                  // src/grid.f:40
@@ -305,22 +313,21 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     for (int d = 0; d < 3; d++) {
         int M, P;
         st::choose_layout(n[d], &M, &P);
-        const double sx1 = 1.0 / leng_of(period[d], n[d]);   // src/grid.f:31-32
+        const double sx1 = 1.0 / leng_of(period[d], n[d], np[d]);   // src/grid.f:31-32
         const double sx2 = sx1 * sx1;
         c->sx1[d] = sx1;
-        if (st::build_dirtab(&c->th[d], n[d], M, P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0, 6.6624e-01, 1.6688e-01) ||
-            (M % 2 == 0 && 2 * P <= 64 &&
-             st::build_dirtab(&c->th[3 + d], n[d], M / 2, 2 * P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0,
-                              6.6624e-01, 1.6688e-01))) {
+        // the narrow layout (M/2 points per lane, 2P lanes) where it exists;
+        // 512-point lines already use 64 lanes and keep M = 8
+        const bool narrow = M % 2 == 0 && 2 * P <= 64;
+        if (st::build_dirtab(&c->th[d], n[d], M, P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0, 6.6624e-01, 1.6688e-01,
+                             np[d]) ||
+            (narrow && st::build_dirtab(&c->th[3 + d], n[d], M / 2, 2 * P, 0, sx1, sx2, 1.0 / 3.0, 2.0 / 11.0,
+                                        6.6624e-01, 1.6688e-01, np[d]))) {
             fail(c, -2, "coefficient tables failed for direction %d", d);
             *out = c;
             return -2;
         }
-        if (M % 2 || 2 * P > 64) {
-            fail(c, -2, "line length %d in direction %d has no 512-thread layout", n[d], d);
-            *out = c;
-            return -2;
-        }
+        if (!narrow) c->th[3 + d] = c->th[d];
     }
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)) != hipSuccess) {
@@ -738,7 +745,10 @@ extern "C" int st_tstep(st_ctx *c, double cfl_resv, int step, double *dt)
 {
     if (!c || !dt) return -1;
     double t = c->p.cfl_cal / cfl_resv;                      // src/lddrk4_periodic_sf.f:96
-    if (c->p.hyper_param == 1 && step > 1) {                  // :99
+    // :99; the non-periodic driver drops the step > 1 guard
+    // (src/lddrk4_nonperiodic_sf.f:105): before the first Hyper_RHS the
+    // coefficients are zero and the limits infinite
+    if (c->p.hyper_param == 1 && (step > 1 || c->g.nonperiodic_x)) {
         double v[3];
         TRY(reduce(c, 1, 3, v));
         const double p2 = (double)0.2f;                       // default-real literal 0.2 (:141)

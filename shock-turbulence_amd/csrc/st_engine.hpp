@@ -106,10 +106,12 @@ struct Walk {
     }
 };
 
-// Per-lane view during the compute phase; P lanes per line.
-template <int M, int P_>
+// Per-lane view during the compute phase; P lanes per line.  NP: the line
+// is non-periodic (closure rows at both ends, no wrap).
+template <int M, int P_, bool NP_ = false>
 struct Lane {
     static constexpr int P = P_;
+    static constexpr bool NP = NP_;
     int c, seg, b;         // chunk index, first lane of the line, line in tile
     double *lds;
     int ss;                // slot size (doubles)
@@ -121,10 +123,10 @@ struct Lane {
 
 // ss = B*M*P = NT*M doubles: a compile-time constant for the kernel, so the
 // slot offsets fold into the LDS instructions' immediate offsets.
-template <int M, int P>
-DEV Lane<M, P> make_lane(double *lds, int ss)
+template <int M, int P, bool NP = false>
+DEV Lane<M, P, NP> make_lane(double *lds, int ss)
 {
-    Lane<M, P> L;
+    Lane<M, P, NP> L;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     L.c = lane & (P - 1);
@@ -136,8 +138,12 @@ DEV Lane<M, P> make_lane(double *lds, int ss)
 }
 
 // Window of M own values plus HALO on each side, fetched from neighbours.
-template <int M, int P>
-DEV void window(const double (&v)[M], const Lane<M, P> &L, double (&w)[M + 2 * HALO])
+// On a non-periodic line the points beyond either end are the mirror
+// images f(-m) = f(m), f(n-1+m) = f(n-1-m) (the symmetry of
+// src/filters_nonperiodic1d.f:830-943); the compact operators overwrite
+// their closure rows and never read them.
+template <int M, int P, bool NP>
+DEV void window(const double (&v)[M], const Lane<M, P, NP> &L, double (&w)[M + 2 * HALO])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) w[HALO + k] = v[k];
@@ -152,6 +158,31 @@ DEV void window(const double (&v)[M], const Lane<M, P> &L, double (&w)[M + 2 * H
         const int dl = 1 + o / M;
         const int idx = o % M;
         w[HALO + M + o] = __shfl(v[idx], L.src(dl));
+    }
+    if constexpr (NP) {
+        constexpr int n = M * P;
+        // chunks whose window reaches past the first point: row -j sits
+        // at w[HALO - c*M - j], its image row j at w[HALO - c*M + j]
+        // (selects, not branches: every index stays a constant)
+#pragma unroll
+        for (int cc = 0; cc * M < HALO; cc++) {
+            const bool here = L.c == cc;
+#pragma unroll
+            for (int j = 1; j <= HALO; j++) {
+                const int at = HALO - cc * M - j;
+                if (at >= 0) w[at] = here ? w[HALO - cc * M + j] : w[at];
+            }
+        }
+        // and past the last: row n-1+j at w[HALO + n-1+j - c*M]
+#pragma unroll
+        for (int cc = P - 1; cc >= 0 && cc * M + M - 1 + HALO > n - 1; cc--) {
+            const bool here = L.c == cc;
+#pragma unroll
+            for (int j = 1; j <= HALO; j++) {
+                const int at = HALO + n - 1 + j - cc * M;
+                if (at < M + 2 * HALO) w[at] = here ? w[HALO + n - 1 - j - cc * M] : w[at];
+            }
+        }
     }
 }
 
@@ -311,8 +342,8 @@ DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c)
 // sides (the scale 1+rho^2 or 1/kappa already folded in) and returns x.
 // Order 1: x = (1 - rho S^-1)^-1 (1 - rho S)^-1 r,
 // order 2: x = (1 - a1 S^-1 - a2 S^-2)^-1 (1 - a1 S - a2 S^2)^-1 r.
-template <int K, int M, int P>
-DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P> &L)
+template <int K, int M, int P, bool NP>
+DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P, NP> &L)
 {
     const double a1 = T.a1, a2 = T.a2;
     const int c = L.c;
@@ -334,8 +365,10 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P> &L)
         if (c == 0) cin[i] = 0.0;
         w[i] = 0.0;
     }
-    madd<K>(T.wrap, tot, w);
-    madd<K>(T.lp[c], w, cin);
+    if constexpr (!NP) {
+        madd<K>(T.wrap, tot, w);
+        madd<K>(T.lp[c], w, cin);
+    }
 #pragma unroll
     for (int k = 0; k < M; k++) {
         r[k] = fma(T.cf[k][0], cin[0], r[k]);
@@ -356,8 +389,10 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P> &L)
         if (c == P - 1) cin[i] = 0.0;
         w[i] = 0.0;
     }
-    madd<K>(T.wrap, tot, w);
-    madd<K>(T.lp[P - 1 - c], w, cin);
+    if constexpr (!NP) {
+        madd<K>(T.wrap, tot, w);
+        madd<K>(T.lp[P - 1 - c], w, cin);
+    }
 #pragma unroll
     for (int k = 0; k < M; k++) {
         r[k] = fma(T.cf[M - 1 - k][0], cin[0], r[k]);
@@ -365,38 +400,166 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P> &L)
     }
 }
 
+// ---- non-periodic lines ---------------------------------------------------
+
+// Value at row ROW of this lane's line after a solve (every lane gets it).
+template <int M, int P, bool NP>
+DEV double line_value(const double (&x)[M], const Lane<M, P, NP> &L, int row)
+{
+    const int src = row / M;
+    if constexpr (P == 64) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x[row % M]), src),
+                                __builtin_amdgcn_readlane(__double2loint(x[row % M]), src));
+    } else {
+        return __shfl(x[row % M], L.seg + src);
+    }
+}
+
+// Boundary correction x -= Z q, q = G yb (the np_* tables of OpTab).
+template <int MB, int M, int P, bool NP>
+DEV void np_fix(double (&x)[M], const OpTab &T, const Lane<M, P, NP> &L)
+{
+    constexpr int n = M * P;
+    double yb[10];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        yb[t] = line_value<M, P>(x, L, t);
+        yb[5 + t] = line_value<M, P>(x, L, n - 5 + t);
+    }
+    double q[MB];
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+        double s = T.npG[j][0] * yb[0];
+#pragma unroll
+        for (int t = 1; t < 10; t++) s = fma(T.npG[j][t], yb[t], s);
+        q[j] = s;
+    }
+    const double *Z = T.npZ + L.c;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+#pragma unroll
+        for (int j = 0; j < MB; j++) x[k] = fma(-q[j], Z[(j * M + k) * P], x[k]);
+    }
+}
+
+// Closure rows of the right-hand side.  OP 0: first derivative
+// (src/derivCDopt6_nonperiodic1d.f:463-471, 494-502, with the closure
+// constants of src/header:173-175), 1: second derivative (:615-629, 653-668),
+// 2: compact filter (src/filters_nonperiodic1d.f:324-349, 384-408, with
+// src/LES_header:118-148).  f(j) reads the j-th point from the near end;
+// the first-derivative rows at the far end are the negated mirror images.
+// The one-sided rows cancel strongly (the second-derivative row 1 sums
+// |coefficients| 56 times its result), so they keep the reference's
+// rounding: every product rounded, sums left to right, no contraction.
+template <int OP, class F>
+DEV double closure_row(int r, const F &f)
+{
+#pragma clang fp contract(off)
+    if constexpr (OP == 0) {
+        constexpr double cl1 = 5.0 / 9.0, cl2 = 1.0 / 18.0, cl3 = 1.0 / 36.0, cl4 = 7.0 / 9.0;
+        if (r == 0) return (-2.5 * f(0) + 2.0 * f(1)) + 5.0e-1 * f(2);
+        if (r == 1) return ((-cl1 * f(0) - 5.0e-1 * f(1)) + f(2)) + cl2 * f(3);
+        return cl3 * (f(4) - f(0)) + cl4 * (f(3) - f(1));
+    } else if constexpr (OP == 1) {
+        constexpr double cl5 = 3.0 / 44.0, cl6 = 12.0 / 11.0;
+        if (r == 0) return ((1.3e1 * f(0) - 2.7e1 * f(1)) + 1.5e1 * f(2)) - f(3);
+        if (r == 1) return (((1.2375 * f(0) - 3.0 * f(1)) + 2.325 * f(2)) - 6.0e-1 * f(3)) + 3.75e-2 * f(4);
+        return cl5 * ((f(0) - 2.0 * f(2)) + f(4)) + cl6 * ((f(1) - 2.0 * f(2)) + f(3));
+    } else {
+        constexpr double ab = 0.45, al = 6.6624e-01, be = 1.6688e-01;
+        if (r == 0)
+            return (((15.0 / 16.0 * f(0) + 1.0 / 4.0 * f(1)) + -3.0 / 8.0 * f(2)) + 1.0 / 4.0 * f(3)) + -1.0 / 16.0 * f(4);
+        if (r == 1) {
+            constexpr double c1 = 1.0 / 16.0 + (7.0 / 8.0) * ab, c2 = 3.0 / 4.0 + (1.0 / 2.0) * ab;
+            constexpr double c3 = 3.0 / 8.0 + (1.0 / 4.0) * ab, c4 = -1.0 / 4.0 + (1.0 / 2.0) * ab;
+            constexpr double c5 = 1.0 / 16.0 - (1.0 / 8.0) * ab;
+            return (((c1 * f(0) + c2 * f(1)) + c3 * f(2)) + c4 * f(3)) + c5 * f(4);
+        }
+        if (r == 2) {
+            constexpr double c0 = 5.0 / 8.0 + (3.0 / 4.0) * ab, c1 = (1.0 / 2.0 + ab) / 2.0;
+            constexpr double c2 = (-1.0 / 8.0 + ab / 4.0) / 2.0;
+            return (((c2 * f(4) + c1 * f(3)) + c0 * f(2)) + c1 * f(1)) + c2 * f(0);
+        }
+        constexpr double c0 = 11.0 / 16.0 + (5.0 / 8.0) * al - (5.0 / 8.0) * be;
+        constexpr double c1 = (15.0 / 32.0 + (17.0 / 16.0) * al + (15.0 / 16.0) * be) / 2.0;
+        constexpr double c2 = (-3.0 / 16.0 + (3.0 / 8.0) * al + (13.0 / 8.0) * be) / 2.0;
+        constexpr double c3 = (1.0 / 32.0 - al / 16.0 + be / 16.0) / 2.0;
+        return (((((c3 * f(6) + c2 * f(5)) + c1 * f(4)) + c0 * f(3)) + c1 * f(2)) + c2 * f(1)) + c3 * f(0);
+    }
+}
+
+// Overwrite the closure rows (the first NB and last NB of the line) of the
+// right-hand side out = K * stencil.  The chunks that hold them are
+// enumerated at compile time so every window index is a constant.
+template <int OP, int M, int P, bool NP>
+DEV void closure_rows(const double (&w)[M + 2 * HALO], double K, const Lane<M, P, NP> &L, double (&out)[M])
+{
+    constexpr int n = M * P, NB = OP == 2 ? 4 : 3;
+    // (selects, not branches: every index stays a constant)
+#pragma unroll
+    for (int cc = 0; cc * M < NB; cc++) {
+        const bool here = L.c == cc;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const int i = cc * M + k;
+            if (i < NB) {
+                const int o = HALO - cc * M;
+                const double v = closure_row<OP>(i, [&](int j) { return w[o + j]; });
+                out[k] = here ? K * v : out[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int cc = P - 1; cc >= 0 && cc * M + M > n - NB; cc--) {
+        const bool here = L.c == cc;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const int i = cc * M + k;
+            if (i >= n - NB) {
+                const int o = HALO + n - 1 - cc * M;
+                const double v = closure_row<OP>(n - 1 - i, [&](int j) { return w[o - j]; });
+                out[k] = here ? K * (OP == 0 ? -v : v) : out[k];
+            }
+        }
+    }
+}
+
 // ---- operators on a window -----------------------------------------------
 
 // First derivative: K = cadd*sx(1)*(N-1)*(1+rho^2).
-template <int M, int P>
+template <int M, int P, bool NP>
 DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M, P> &L, double (&out)[M])
+               const Lane<M, P, NP> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
         const int i = k + HALO;
         out[k] = K * ((C.c1 * (w[i + 3] - w[i - 3]) + C.b1 * (w[i + 2] - w[i - 2])) + C.a1 * (w[i + 1] - w[i - 1]));
     }
+    if constexpr (NP) closure_rows<0>(w, K, L, out);
     solve<1, M>(out, T, L);
+    if constexpr (NP) np_fix<4>(out, T, L);
 }
 
 // Second derivative: K = (N-1)^2*sx(2)*(1+rho^2).
-template <int M, int P>
+template <int M, int P, bool NP>
 DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M, P> &L, double (&out)[M])
+               const Lane<M, P, NP> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
         const int i = k + HALO;
         out[k] = K * (((C.c2 * (w[i + 3] + w[i - 3]) + C.b2 * (w[i + 2] + w[i - 2])) + C.a2 * (w[i + 1] + w[i - 1])) - C.d2 * w[i]);
     }
+    if constexpr (NP) closure_rows<1>(w, K, L, out);
     solve<1, M>(out, T, L);
+    if constexpr (NP) np_fix<4>(out, T, L);
 }
 
 // Eighth-order compact filter: K = 1/kappa.
-template <int M, int P>
+template <int M, int P, bool NP>
 DEV void op_filt(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-                 const Lane<M, P> &L, double (&out)[M])
+                 const Lane<M, P, NP> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -404,7 +567,9 @@ DEV void op_filt(const double (&w)[M + 2 * HALO], const Coef &C, double K, const
         out[k] = K * ((((C.f4 * (w[i + 4] + w[i - 4]) + C.f3 * (w[i + 3] + w[i - 3])) + C.f2 * (w[i + 2] + w[i - 2]))
                        + C.f1 * (w[i + 1] + w[i - 1])) + C.f0 * w[i]);
     }
+    if constexpr (NP) closure_rows<2>(w, K, L, out);
     solve<2, M>(out, T, L);
+    if constexpr (NP) np_fix<6>(out, T, L);
 }
 
 // Truncated Gaussian (explicit).
@@ -425,8 +590,12 @@ struct Prim {
     double vs, u, v, w, p, T;
 };
 
+// Rounded as the reference rounds it (no contraction): the temperature
+// feeds second derivatives whose one-sided boundary rows cancel strongly,
+// so an ulp of difference here would show up there.
 DEV Prim prim(const Phys &ph, double q1, double q2, double q3, double q4, double q5)
 {
+#pragma clang fp contract(off)
     Prim r;
     r.vs = 1.0 / q4;
     const double ke = q1 * q1 + q2 * q2 + q3 * q3;

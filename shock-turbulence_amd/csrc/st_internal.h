@@ -24,6 +24,18 @@ struct OpTab {
     double sc[6][4];      // A^(M*2^j), j = 0..log2(P)-1 (scan steps, row-major 2x2)
     double wrap[4];       // (I - A^n)^-1
     double lp[64][4];     // A^(M*c), c = 0..P-1 (per-lane injection of the cyclic total)
+    // Non-periodic lines (x of the shock-tube link set).  The first and
+    // last rows of the system are boundary closures; the recurrences run
+    // without the cyclic wrap and so invert Ah = kappa L U (L, U the
+    // recurrence matrices truncated to the line).  The closure matrix is
+    // A = Ah + sum_j e_(i_j) v_j^T over the np_m boundary rows i_j, and
+    //   x = y - Z q,  q = G yb,  y = Ah^-1 b,
+    // with Z = Ah^-1 [e_(i_j)] and G = (I + V^T Z)^-1 V^T restricted to the
+    // ten boundary values yb = (y_0..y_4, y_(n-5)..y_(n-1)) (Woodbury).
+    int np_m;             // boundary rows (4: first/second derivative, 6: filter)
+    int pad2_;
+    double npG[6][10];
+    double npZ[6 * 512];  // column j at row c*M + k: npZ[(j*M + k)*P + c]
 };
 
 // Per-direction operator tables and stencil factors.
@@ -32,6 +44,8 @@ struct DirTab {
     double s1;            // sx(1)*(N-1)        (src/derivCDopt6_periodic.f:181)
     double s2;            // (N-1)^2*sx(2)      (src/derivCDopt6_periodic.f:298)
     int n, M, P, B;       // line length, points per lane, lanes per line, lines per tile
+    int nonper;           // 1: closure rows at both ends, no wrap (the tables' np_* fields)
+    int pad_;
 };
 
 // Compact-scheme coefficients (src/header:191-203, src/LES_header:104-157).
@@ -55,8 +69,11 @@ struct Phys {
 
 // Fill the three operator tables of one direction for a line of n points
 // split into P lanes of M points.  Returns 0 on success.
+// nonper = 1 adds the boundary-closure tables of the non-periodic x
+// operators (src/derivCDopt6_nonperiodic1d.f:41-357,
+// src/filters_nonperiodic1d.f:45-215).
 int build_dirtab(DirTab *t, int n, int M, int P, int B, double sx1, double sx2,
-                 double d1_alpha, double d2_alpha, double f_alpha, double f_beta);
+                 double d1_alpha, double d2_alpha, double f_alpha, double f_beta, int nonper = 0);
 
 // Choose (M, P) for a line length; returns 0 if n is unsupported.
 int choose_layout(int n, int *M, int *P);

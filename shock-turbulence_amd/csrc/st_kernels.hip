@@ -21,7 +21,8 @@ namespace st {
 // Viscous terms built from products of first derivatives
 // (src/rhsv_periodic_sf.f:1128-1154 without the mixed and own-direction
 // second derivatives, and :1252-1275), plus |S| (src/hyper_dynamic.f:12-33,
-// 227-245) and the dilatation sensor C_beta (src/hyper_dynamic.f:142-160).
+// 227-245) and the dilatation sensor C_beta (src/hyper_dynamic.f:142-160;
+// skipped when Cb is null: the constant-coefficient src/hyper_routines.f).
 __global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
 {
     const long N = a.N;
@@ -55,10 +56,12 @@ __global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
         if (hyp) {
             const double s12 = 0.5 * (gxy + gyx), s13 = 0.5 * (gxz + gzx), s23 = 0.5 * (gyz + gzy);
             a.S[gi] = sqrt(0.5 * (gxx * gxx + gyy * gyy + gzz * gzz + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
-            const double wx = gzy - gyz, wy = gxz - gzx, wz = gyx - gxy;
-            const double om2 = wx * wx + wy * wy + wz * wz;
-            const double fac = 0.5 * (1.0 - tanh(2.5 + 10.0 * dil));
-            a.Cb[gi] = fac * dil * dil / (dil * dil + om2 + 1.0e-7) + 1.0e-20;
+            if (a.Cb) {
+                const double wx = gzy - gyz, wy = gxz - gzx, wz = gyx - gxy;
+                const double om2 = wx * wx + wy * wy + wz * wz;
+                const double fac = 0.5 * (1.0 - tanh(2.5 + 10.0 * dil));
+                a.Cb[gi] = fac * dil * dil / (dil * dil + om2 + 1.0e-7) + 1.0e-20;
+            }
         }
     }
 }

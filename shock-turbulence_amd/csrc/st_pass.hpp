@@ -27,7 +27,7 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // (read-modify-write operands): prefetch() loads them during the load phase,
 // which visits the same points per thread as the store phase, and they
 // reach store_early() (REUSE) or store() appended after the results.
-template <int M, int P, int NT, class PASS>
+template <int M, int P, int NT, bool NP, class PASS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
 k_pass(Geo g, PassArgs a)
 {
@@ -59,9 +59,9 @@ k_pass(Geo g, PassArgs a)
     __syncthreads();
 
     // 2. compute (line mapping)
-    const Lane<M, P> L = make_lane<M, P>(lds, ss);
+    const Lane<M, P, NP> L = make_lane<M, P, NP>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
-    PASS::template compute<M, P>(g, a, L, out);
+    PASS::template compute<M, P, NP>(g, a, L, out);
 
     // 3. store (through LDS, memory order)
     __syncthreads();
@@ -122,8 +122,8 @@ struct NoReuse {
 // ---------------------------------------------------------------------
 DEV double qf(const PassArgs &a, int comp, long gi) { return a.Q[comp * a.N + gi]; }
 
-template <int M, int P>
-DEV void qown(const Lane<M, P> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
+template <int M, int P, bool NP>
+DEV void qown(const Lane<M, P, NP> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
 {
     q1 = L.ld(0, k);
     q2 = L.ld(1, k);
@@ -141,50 +141,50 @@ DEV void upd(double *p, double v, bool first)
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-template <int M, int P, int NT, class PASS>
+template <int M, int P, int NT, bool NP, class PASS>
 static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
 {
     const size_t lds = (size_t)pass_slots<PASS>() * NT * M * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static size_t lds_set = 0;
     if (lds > 64 * 1024 && lds > lds_set) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, P, NT, PASS>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, P, NT, NP, PASS>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL((k_pass<M, P, NT, PASS>), dim3((unsigned)g.ntiles), dim3(NT), lds, s, g, a);
+    hipLaunchKernelGGL((k_pass<M, P, NT, NP, PASS>), dim3((unsigned)g.ntiles), dim3(NT), lds, s, g, a);
     return hipGetLastError();
 }
 
 // The lane layouts (M points per lane, P lanes per line) that
 // choose_layout and its narrow variant (M/2, 2P) produce for lines of
 // 16..512 points.
-template <class PASS, int NT>
+template <class PASS, int NT, bool NP>
 static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStream_t s)
 {
     if constexpr (NT == 1024) {
         // 16 or 32 lines per tile: lanes of 32 or 64 per line only
         switch (M * 100 + P) {
-        case 432: return launch_t<4, 32, NT, PASS>(g, a, s);
-        case 464: return launch_t<4, 64, NT, PASS>(g, a, s);
-        case 832: return launch_t<8, 32, NT, PASS>(g, a, s);
-        case 864: return launch_t<8, 64, NT, PASS>(g, a, s);
+        case 432: return launch_t<4, 32, NT, NP, PASS>(g, a, s);
+        case 464: return launch_t<4, 64, NT, NP, PASS>(g, a, s);
+        case 832: return launch_t<8, 32, NT, NP, PASS>(g, a, s);
+        case 864: return launch_t<8, 64, NT, NP, PASS>(g, a, s);
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (M * 100 + P) {
-        case 116: return launch_t<1, 16, NT, PASS>(g, a, s);
-        case 208: return launch_t<2, 8, NT, PASS>(g, a, s);
-        case 216: return launch_t<2, 16, NT, PASS>(g, a, s);
-        case 408: return launch_t<4, 8, NT, PASS>(g, a, s);
-        case 416: return launch_t<4, 16, NT, PASS>(g, a, s);
-        case 432: return launch_t<4, 32, NT, PASS>(g, a, s);
-        case 464: return launch_t<4, 64, NT, PASS>(g, a, s);
-        case 808: return launch_t<8, 8, NT, PASS>(g, a, s);
-        case 816: return launch_t<8, 16, NT, PASS>(g, a, s);
-        case 832: return launch_t<8, 32, NT, PASS>(g, a, s);
-        case 864: return launch_t<8, 64, NT, PASS>(g, a, s);
+        case 116: return launch_t<1, 16, NT, NP, PASS>(g, a, s);
+        case 208: return launch_t<2, 8, NT, NP, PASS>(g, a, s);
+        case 216: return launch_t<2, 16, NT, NP, PASS>(g, a, s);
+        case 408: return launch_t<4, 8, NT, NP, PASS>(g, a, s);
+        case 416: return launch_t<4, 16, NT, NP, PASS>(g, a, s);
+        case 432: return launch_t<4, 32, NT, NP, PASS>(g, a, s);
+        case 464: return launch_t<4, 64, NT, NP, PASS>(g, a, s);
+        case 808: return launch_t<8, 8, NT, NP, PASS>(g, a, s);
+        case 816: return launch_t<8, 16, NT, NP, PASS>(g, a, s);
+        case 832: return launch_t<8, 32, NT, NP, PASS>(g, a, s);
+        case 864: return launch_t<8, 64, NT, NP, PASS>(g, a, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -192,7 +192,8 @@ static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStr
 
 // t holds the tables of the chosen layout (M points per lane, P lanes per
 // line); nt threads per workgroup give B = nt/P lines per tile.
-template <class PASS>
+// XL = false for passes that never sweep x (no non-periodic instances).
+template <class PASS, bool XL = true>
 static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     Geo g;
@@ -208,9 +209,18 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
     const long lot = g.npts / g.n;
     if (lot % g.B) return hipErrorInvalidValue;
     g.ntiles = lot / g.B;
-    if (nt == 256) return launch_m<PASS, 256>(t.M, t.P, g, a, s);
-    if (nt == 512) return launch_m<PASS, 512>(t.M, t.P, g, a, s);
-    if (nt == 1024) return launch_m<PASS, 1024>(t.M, t.P, g, a, s);
+    // non-periodic lines are the x lines of the shock-tube link set
+    if (t.nonper && dir == 0) {
+        if constexpr (XL) {
+            if (nt == 256) return launch_m<PASS, 256, true>(t.M, t.P, g, a, s);
+            if (nt == 512) return launch_m<PASS, 512, true>(t.M, t.P, g, a, s);
+            if (nt == 1024) return launch_m<PASS, 1024, true>(t.M, t.P, g, a, s);
+        }
+        return hipErrorInvalidValue;
+    }
+    if (nt == 256) return launch_m<PASS, 256, false>(t.M, t.P, g, a, s);
+    if (nt == 512) return launch_m<PASS, 512, false>(t.M, t.P, g, a, s);
+    if (nt == 1024) return launch_m<PASS, 1024, false>(t.M, t.P, g, a, s);
     return hipErrorInvalidValue;
 }
 

@@ -31,8 +31,8 @@ struct PassFilterQ {
             }
         }
     }
-    template <int M, int P>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
     {
         const OpTab &T = a.tab->op[2];
 #pragma unroll 1
@@ -73,8 +73,8 @@ hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int
 {
     switch (kind) {
     case K_RKF_X: return launch<PassFilterQ<0>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_F_Y: return launch<PassFilterQ<1>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_F_Z: return launch<PassFilterQ<2>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_F_Y: return launch<PassFilterQ<1>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_F_Z: return launch<PassFilterQ<2>, false>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }

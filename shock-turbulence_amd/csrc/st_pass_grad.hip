@@ -55,8 +55,8 @@ struct PassGrad {
     // the Laplacian of e; then each velocity, whose slot takes its
     // derivative.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations.
-    template <int M, int P>
-    DEV static void compute_wide(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    template <int M, int P, bool NP>
+    DEV static void compute_wide(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -148,8 +148,8 @@ struct PassGrad {
     // the Laplacian of e.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations, and each loop iteration
     // keeps one window and one solve live at a time.
-    template <int M, int P>
-    DEV static void compute_lean(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    template <int M, int P, bool NP>
+    DEV static void compute_lean(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -244,11 +244,11 @@ struct PassGrad {
         }
     }
 
-    template <int M, int P>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P> &L, double (&out)[5][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
     {
-        if constexpr (LEAN) compute_lean<M, P>(g, a, L, out);
-        else compute_wide<M, P>(g, a, L, out);
+        if constexpr (LEAN) compute_lean<M, P, NP>(g, a, L, out);
+        else compute_wide<M, P, NP>(g, a, L, out);
     }
 
     DEV static void store_early(const Geo &g, const PassArgs &a, long gi, const double *v)
@@ -271,7 +271,7 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
     switch (kind) {
     case K_GRAD:
         if (nt == 1024) return launch<PassGrad<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
-        if (dir == 1) return launch<PassGrad<3, false>>(t, nt, dir, My, Mz, Mx, a, s);
+        if (dir == 1) return launch<PassGrad<3, false>, false>(t, nt, dir, My, Mz, Mx, a, s);
         return launch<PassGrad<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }

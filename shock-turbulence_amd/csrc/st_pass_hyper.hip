@@ -12,8 +12,8 @@ template <bool LAST>
 struct PassLap2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
-    template <int M, int P>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[1][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -50,8 +50,8 @@ struct PassGaussEta : NoReuse {
         v[0] = a.eta[gi];
         v[1] = a.eta[a.N + gi];
     }
-    template <int M, int P>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&out)[2][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[2][M])
     {
 #pragma unroll
         for (int m = 0; m < 2; m++) {
@@ -70,7 +70,8 @@ struct PassGaussEta : NoReuse {
             return;
         }
         const Phys &ph = a.ph;
-        a.Hc[gi] = a.Cb[gi] * v[0] * ph.dx6;            // beta*
+        // beta*; constant C_beta = 1 without the sensor (src/hyper_routines.f:99-101,204)
+        a.Hc[gi] = (a.Cb ? a.Cb[gi] : 1.0) * v[0] * ph.dx6;
         a.Hc[a.N + gi] = ph.cmu * v[0] * ph.dx6;        // mu*
         a.Hc[2 * a.N + gi] = ph.ck * v[1] * ph.dx5;     // kappa*
     }
@@ -127,8 +128,8 @@ struct PassHyperDiv {
         v[2] = tz;
         v[3] = uh * tx + vh * ty + wh * tz + kap * G[(9 + g.dir) * N + gi];
     }
-    template <int M, int P>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
     {
 #pragma unroll 1
         for (int m = 0; m < 4; m++) {
@@ -156,9 +157,9 @@ hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int 
 {
     switch (kind) {
     case K_LAP2: return launch<PassLap2<false>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_LAP2_LAST: return launch<PassLap2<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_LAP2_LAST: return launch<PassLap2<true>, false>(t, nt, dir, My, Mz, Mx, a, s);
     case K_GAUSS_ETA: return launch<PassGaussEta<false>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>, false>(t, nt, dir, My, Mz, Mx, a, s);
     case K_HDIV: return launch<PassHyperDiv>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }

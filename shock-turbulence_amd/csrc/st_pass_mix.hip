@@ -58,8 +58,8 @@ struct PassMix {
             v[NMIX + 1] = a.L2e[gi];
         }
     }
-    template <int M, int P>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P> &L, double (&)[1][M])
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
     {
         // each result replaces its own input in LDS
 #pragma unroll 1
@@ -120,11 +120,11 @@ struct PassMix {
 hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     switch (kind) {
-    case K_MIX_Y: return launch<PassMix<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_MIX_Z: return launch<PassMix<2, false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIX_Y: return launch<PassMix<1, false>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIX_Z: return launch<PassMix<2, false>, false>(t, nt, dir, My, Mz, Mx, a, s);
     case K_MIXH_X: return launch<PassMix<0, true>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_MIXH_Y: return launch<PassMix<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_MIXH_Z: return launch<PassMix<2, true>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIXH_Y: return launch<PassMix<1, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_MIXH_Z: return launch<PassMix<2, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -12,6 +12,7 @@ module st_capi
      integer(c_int) :: nx, ny, nz
      real(c_double) :: period_x, period_y, period_z
      real(c_double) :: x_start, y_start, z_start
+     integer(c_int) :: nonperiodic_x = 0
   end type st_grid
 
   type, bind(C) :: st_params

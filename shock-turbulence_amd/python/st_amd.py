@@ -29,7 +29,8 @@ class StError(RuntimeError):
 class Grid(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("period_x", ctypes.c_double), ("period_y", ctypes.c_double), ("period_z", ctypes.c_double),
-                ("x_start", ctypes.c_double), ("y_start", ctypes.c_double), ("z_start", ctypes.c_double)]
+                ("x_start", ctypes.c_double), ("y_start", ctypes.c_double), ("z_start", ctypes.c_double),
+                ("nonperiodic_x", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
@@ -135,8 +136,10 @@ class Solver:
     """One device context (grid + parameters + resident state)."""
 
     def __init__(self, nx, ny, nz, period=(2 * np.pi,) * 3, Re=100.0, Pr=0.7, gamma=1.4, vis_exp=0.76,
-                 bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5, d2_mode=0, rk_scheme=0, start=(0.0, 0.0, 0.0)):
-        self.grid = Grid(nx, ny, nz, period[0], period[1], period[2], start[0], start[1], start[2])
+                 bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5, d2_mode=0, rk_scheme=0, start=(0.0, 0.0, 0.0),
+                 nonperiodic_x=0):
+        self.grid = Grid(nx, ny, nz, period[0], period[1], period[2], start[0], start[1], start[2],
+                         1 if nonperiodic_x else 0)
         self.params = Params(Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal, d2_mode, rk_scheme)
         self.shape = (ny, nz, nx)
         h = ctypes.c_void_p()
