@@ -182,7 +182,8 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
     // the widest configured tile that the line count and layout allow
     int nt = c->cfg_nt[kind][dir];
     const long lot = c->N / t.n;
-    while (nt > 256 && ((nt == 1024 && t.P < 32) || lot % (nt / t.P))) nt /= 2;
+    // (1024-thread tiles exist for M = 4, 8 with 32 or 64 lanes per line)
+    while (nt > 256 && ((nt == 1024 && (t.P < 32 || (t.M & (t.M - 1)))) || lot % (nt / t.P))) nt /= 2;
     hipError_t err = st::run_pass(kind, t, nt, dir, c->My, c->Mz, c->Mx, a, c->s);
     if (c->prof) {
         prof_end(c, e);
@@ -250,7 +251,8 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     for (int d = 0; d < 3; d++) {
         int M, P;
         if (n[d] < 16 || !st::choose_layout(n[d], &M, &P)) {
-            fail(c, -2, "unsupported line length %d in direction %d (need n = M*P, M in {2,4,6,8}, P in {8,16,32,64})", n[d], d);
+            fail(c, -2, "unsupported line length %d in direction %d (need n = M*P, M in {2,4,6,8}, P in {8,16,32,64}: "
+                        "16..512 points, powers of two or three times one)", n[d], d);
             *out = c;
             return -2;
         }

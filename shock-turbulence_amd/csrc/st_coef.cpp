@@ -236,7 +236,7 @@ int fill_np(OpTab *t, int op, double alpha, double beta, int n, int M, int P)
 
 int choose_layout(int n, int *M, int *P)
 {
-    static const int cand[] = {8, 4, 2};
+    static const int cand[] = {8, 6, 4, 2};
     for (int m : cand) {
         if (n % m) continue;
         const int p = n / m;

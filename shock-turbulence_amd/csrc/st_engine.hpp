@@ -48,11 +48,19 @@ DEV long tile_of(const Geo &g)
 // n doubles; point s = c*M + k sits at row k, column c.  Rows are rotated by
 // b and columns XOR-swizzled so that lane-parallel reads at fixed k and the
 // memory-order staging writes both spread over the 32 double-wide banks.
+// (M = 3 or 6: rows rotate by b mod M; any rotation and swizzle is a
+// bijection of the block, only the bank spread depends on them.)
 template <int M>
 HD int lds_idx(int b, int k, int c, int P)
 {
-    const int f = k * (P / M) + (M == 4 ? 4 * b : b);
-    return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
+    if constexpr ((M & (M - 1)) == 0) {
+        const int f = k * (P / M) + (M == 4 ? 4 * b : b);
+        return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
+    } else {
+        int r = k + b % M;
+        r = r >= M ? r - M : r;
+        return b * (M * P) + r * P + (c ^ ((k + b) & (P - 1)));
+    }
 }
 
 // Memory-order walk of one tile by one thread.  The tile holds B = NT/P
@@ -97,7 +105,7 @@ struct Walk {
         if (g.dir == 1) {
             const int e = threadIdx.x + NT * r;
             bb = e / n;
-            ss = e & (n - 1);
+            ss = e - bb * n;
         } else {
             bb = b;
             ss = s0 + r * P;

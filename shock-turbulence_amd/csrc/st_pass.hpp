@@ -159,7 +159,7 @@ static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
 
 // The lane layouts (M points per lane, P lanes per line) that
 // choose_layout and its narrow variant (M/2, 2P) produce for lines of
-// 16..512 points.
+// 16..512 points (powers of two) and 48..384 (three times a power of two).
 template <class PASS, int NT, bool NP>
 static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStream_t s)
 {
@@ -185,6 +185,14 @@ static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStr
         case 816: return launch_t<8, 16, NT, NP, PASS>(g, a, s);
         case 832: return launch_t<8, 32, NT, NP, PASS>(g, a, s);
         case 864: return launch_t<8, 64, NT, NP, PASS>(g, a, s);
+        // lines of 48, 96, 192 and 384 points
+        case 316: return launch_t<3, 16, NT, NP, PASS>(g, a, s);
+        case 332: return launch_t<3, 32, NT, NP, PASS>(g, a, s);
+        case 364: return launch_t<3, 64, NT, NP, PASS>(g, a, s);
+        case 608: return launch_t<6, 8, NT, NP, PASS>(g, a, s);
+        case 616: return launch_t<6, 16, NT, NP, PASS>(g, a, s);
+        case 632: return launch_t<6, 32, NT, NP, PASS>(g, a, s);
+        case 664: return launch_t<6, 64, NT, NP, PASS>(g, a, s);
         default: return hipErrorInvalidValue;
         }
     }

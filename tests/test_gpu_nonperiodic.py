@@ -38,7 +38,8 @@ PER = (2.0, 2 * np.pi, 2 * np.pi)
 
 # x lines of 16..512 points: lanes per line P = 8, 16, 32, 64 and both
 # per-lane widths of each pass
-SHAPES = [(16, 32, 16), (32, 16, 16), (64, 16, 16), (128, 16, 32), (256, 16, 16), (512, 16, 16)]
+SHAPES = [(16, 32, 16), (32, 16, 16), (64, 16, 16), (128, 16, 32), (256, 16, 16), (512, 16, 16),
+          (48, 16, 16), (192, 16, 16)]
 
 
 def _log(*a):
@@ -104,7 +105,7 @@ def test_np_polynomial_exact(solvers, shape):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 16, 16), (128, 16, 32), (256, 16, 16)])
+@pytest.mark.parametrize("shape", [(32, 16, 16), (128, 16, 32), (256, 16, 16), (96, 48, 16)])
 def test_np_rhs(solvers, shape, hyper):
     _log("np rhs", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)

@@ -29,9 +29,12 @@ def _log(*a):
 
 
 # Line lengths 16..256 cover every lane layout of the passes: lines of
-# P = 8, 16, 32 and 64 lanes (one to eight lines per wave).
-SHAPES = [(32, 32, 32), (64, 32, 16), (16, 64, 32), (256, 16, 16), (16, 128, 32), (16, 16, 256)]
+# P = 8, 16, 32 and 64 lanes (one to eight lines per wave).  Lines of 48,
+# 96, 192 and 384 points use 6 (or 3) points per lane.
+SHAPES = [(32, 32, 32), (64, 32, 16), (16, 64, 32), (256, 16, 16), (16, 128, 32), (16, 16, 256),
+          (48, 96, 16), (16, 48, 192), (384, 16, 16)]
 LONG = [(256, 16, 16), (16, 128, 32), (32, 16, 256)]
+THREE = [(48, 48, 48), (96, 16, 48), (16, 192, 16)]
 
 
 @pytest.fixture(scope="module")
@@ -86,7 +89,7 @@ def test_d1_sinusoid(solvers, shape):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG)
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG + THREE)
 def test_rhs(solvers, shape, hyper):
     _log("rhs", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
@@ -138,7 +141,7 @@ def test_reductions_and_fix(solvers):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32)] + LONG)
+@pytest.mark.parametrize("shape", [(32, 32, 32)] + LONG + THREE[:2])
 def test_stage(solvers, shape, hyper):
     _log("stage", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
@@ -169,6 +172,12 @@ def _run_steps(solvers, shape, nsteps, tol, **kw):
 def test_steps_20(solvers):
     re = st_amd.re_for_taylor(40.0, 4.0, 1.0, 0.3)
     _run_steps(solvers, (32, 32, 32), 20, 1e-10, Re=re, Pr=0.7, hyper_param=1)
+
+
+def test_steps_20_48(solvers):
+    """The smallest served and batched grid (48^3: 6 points per lane)."""
+    re = st_amd.re_for_taylor(40.0, 4.0, 1.0, 0.3)
+    _run_steps(solvers, (48, 48, 48), 20, 1e-10, Re=re, Pr=0.7, hyper_param=1)
 
 
 @pytest.mark.slow
