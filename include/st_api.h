@@ -119,6 +119,30 @@ int st_profile(st_ctx *ctx, int on);
 int st_profile_read(st_ctx *ctx, int nmax, int *count, double *total_ms, double *fields_rw);
 const char *st_pass_name(int kind);
 
+/* ---- batches of same-size instances (no counterpart in the reference,
+ * which runs one problem per job).  For ensemble and serving callers: a
+ * batch holds nslots instances of one grid and parameter set, stacked field
+ * by field on the device, and advances every loaded slot with one launch
+ * per pass.  Each slot keeps its own state, Re, dt, time and step counter;
+ * dt is computed on the device (Compute_CFL + Compute_Tstep per slot), so a
+ * step needs no host round trip and is replayed from a captured HIP graph.
+ * A slot gives bit-identical results alone and among others.  The
+ * single-instance entry points above act on slot 0. */
+int st_batch_create(const st_grid *grid, const st_params *params, int nslots, st_ctx **out);
+int st_batch_slots(const st_ctx *ctx);
+/* Load a slot's state and run settings and mark it active: it advances on
+ * each st_batch_march step while step < step0 + nsteps (the driver's step
+ * counter starts at step0, src/main.f:129-136); Re <= 0 means inviscid. */
+int st_slot_load(st_ctx *ctx, int slot, const double *Q, double Re, double time0, int step0, int nsteps);
+int st_slot_store(st_ctx *ctx, int slot, double *Q);
+int st_slot_release(st_ctx *ctx, int slot);
+/* Enqueue nsteps batch steps on the context's stream and return (st_sync or
+ * st_batch_status waits).  Finished and released slots do no work. */
+int st_batch_march(st_ctx *ctx, int nsteps);
+/* Per-slot step counter, time, last dt and active flag (arrays of nslots;
+ * any may be NULL); waits for the stream. */
+int st_batch_status(st_ctx *ctx, int *step, double *time, double *dt_last, int *active);
+
 /* Seeded isotropic-turbulence initial condition (new; the reference restarts
  * from a file, src/main.f:69-71).  Q(My,Mz,Mx,5) on the host. */
 int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,

@@ -36,7 +36,9 @@ struct st_ctx {
     st_grid g;
     st_params p;
     int Mx, My, Mz;
-    long N;
+    int ns;                         // instances (slots) stacked per field
+    long ni;                        // points per instance
+    long N;                         // field stride: ns * ni
     double sx1[3];
     st::Phys ph;
     st::Coef C;
@@ -48,6 +50,14 @@ struct st_ctx {
     double *dQp, *L2ep;             // z and x partials of the first-derivative passes
     double *red = nullptr;          // reduction partials (device)
     double *red_h = nullptr;        // pinned host copy
+    long red_n = 0;                 // doubles in red / red_h
+    // per-slot state of a batch (device): active flag, run flag of the
+    // current step, step target, driver step counter, 1/Re, dt, time
+    int *s_act = nullptr, *s_run, *s_target, *s_step;
+    double *s_re = nullptr, *s_dt, *s_time, *s_dtlast;
+    std::vector<int> act_h;
+    hipGraphExec_t gexec = nullptr; // one batch step, captured
+    int use_graph = 1;
     hipStream_t s = nullptr;
     char err[512];
     int prof = 0;
@@ -125,6 +135,8 @@ st::PassArgs base_args(st_ctx *c)
     a.C = c->C;
     a.ph = c->ph;
     a.N = c->N;
+    a.ni = c->ni;
+    a.nslot = 1;                    // single-instance entry points act on slot 0
     a.Q = c->Q;
     a.QW = c->QW;
     a.dQ = c->dQ;
@@ -181,7 +193,7 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
     if (c->prof) prof_begin(c, pk, e);
     // the widest configured tile that the line count and layout allow
     int nt = c->cfg_nt[kind][dir];
-    const long lot = c->N / t.n;
+    const long lot = c->ni / t.n;
     // (1024-thread tiles exist for M = 4, 8 with 32 or 64 lanes per line)
     while (nt > 256 && ((nt == 1024 && (t.P < 32 || (t.M & (t.M - 1)))) || lot % (nt / t.P))) nt /= 2;
     hipError_t err = st::run_pass(kind, t, nt, dir, c->My, c->Mz, c->Mx, a, c->s);
@@ -206,9 +218,9 @@ static double leng_of(double period, int n, int nonper = 0)
     return nonper ? period : period * (double)(n - 1) / (double)n;
 }
 
-extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out)
+extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int nslots, st_ctx **out)
 {
-    if (!grid || !params || !out) return -1;
+    if (!grid || !params || !out || nslots < 1) return -1;
     *out = nullptr;
     st_ctx *c = new st_ctx();
     c->err[0] = 0;
@@ -217,7 +229,10 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     c->Mx = grid->nx;
     c->My = grid->ny;
     c->Mz = grid->nz;
-    c->N = (long)c->Mx * c->My * c->Mz;
+    c->ns = nslots;
+    c->ni = (long)c->Mx * c->My * c->Mz;
+    c->N = c->ni * nslots;
+    c->act_h.assign(nslots, 0);
     std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
     std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
     std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
@@ -257,7 +272,7 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
             return -2;
         }
         const int B = 256 / P;             // lines per tile (same for the 512-thread layout)
-        const long lot = c->N / n[d];
+        const long lot = c->ni / n[d];
         if (lot % B) {
             fail(c, -2, "line count %ld of direction %d not a multiple of the tile", lot, d);
             *out = c;
@@ -338,14 +353,28 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
         return -3;
     }
     const long nf = 5 + 5 + 5 + 12 + 1 + 1 + 1 + 1 + 1 + 1 + 2 + 3 + 5 + 1 + 1 + 10 + 2;
+    // reduction partials: up to 7 values per block and slot, for the CFL
+    // and dt-limit folds side by side (2 + 3 values)
+    c->red_n = (long)st::reduce_blocks(c->ni) * nslots * 8;
+    void *sp = nullptr;
+    const size_t ns = (size_t)nslots;
     if ((e = hipMalloc(&c->buf, sizeof(double) * nf * c->N)) != hipSuccess ||
         (e = hipMalloc(&c->td, sizeof(st::DirTab) * 6)) != hipSuccess ||
-        (e = hipMalloc(&c->red, sizeof(double) * 1024 * 8)) != hipSuccess ||
-        (e = hipHostMalloc(&c->red_h, sizeof(double) * 1024 * 8)) != hipSuccess) {
+        (e = hipMalloc(&c->red, sizeof(double) * c->red_n)) != hipSuccess ||
+        (e = hipHostMalloc(&c->red_h, sizeof(double) * c->red_n)) != hipSuccess ||
+        (e = hipMalloc(&sp, ns * (4 * sizeof(int) + 4 * sizeof(double)))) != hipSuccess) {
         fail(c, -3, "device allocation: %s", hipGetErrorString(e));
         *out = c;
         return -3;
     }
+    c->s_re = (double *)sp;
+    c->s_dt = c->s_re + ns;
+    c->s_time = c->s_dt + ns;
+    c->s_dtlast = c->s_time + ns;
+    c->s_act = (int *)(c->s_dtlast + ns);
+    c->s_run = c->s_act + ns;
+    c->s_target = c->s_run + ns;
+    c->s_step = c->s_target + ns;
     double *p = c->buf;
     const long N = c->N;
     c->Q = p; p += 5 * N;
@@ -367,6 +396,7 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     c->L2ep = p; p += 2 * N;
     if ((e = hipMemcpyAsync(c->td, c->th, sizeof(st::DirTab) * 6, hipMemcpyHostToDevice, c->s)) != hipSuccess ||
         (e = st::run_fill(c->buf, nf * N, 0.0, c->s)) != hipSuccess ||
+        (e = st::run_fill(c->s_re, 4 * (long)ns + 2 * (long)ns, 0.0, c->s)) != hipSuccess ||
         (e = hipStreamSynchronize(c->s)) != hipSuccess) {
         fail(c, -3, "device init: %s", hipGetErrorString(e));
         *out = c;
@@ -374,6 +404,11 @@ extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ct
     }
     *out = c;
     return 0;
+}
+
+extern "C" int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out)
+{
+    return st_batch_create(grid, params, 1, out);
 }
 
 extern "C" void st_ctx_destroy(st_ctx *c)
@@ -385,6 +420,8 @@ extern "C" void st_ctx_destroy(st_ctx *c)
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->s_re) (void)hipFree(c->s_re);
     if (c->buf) (void)hipFree(c->buf);
     if (c->td) (void)hipFree(c->td);
     if (c->red) (void)hipFree(c->red);
@@ -431,25 +468,32 @@ int rk_loop(int scheme, int loop, const double **a, const double **b)
     return 0;
 }
 
-int up(st_ctx *c, double *dev, const double *host, long nfields)
+// Host arrays hold one instance (nfields x ni, contiguous); on the device
+// the fields of slot `slot` sit at dev + f*N + slot*ni.
+int up(st_ctx *c, double *dev, const double *host, long nfields, int slot = 0)
 {
-    HIPCHK(c, hipMemcpyAsync(dev, host, sizeof(double) * nfields * c->N, hipMemcpyHostToDevice, c->s));
+    const size_t w = sizeof(double) * c->ni;
+    HIPCHK(c, hipMemcpy2DAsync(dev + (size_t)slot * c->ni, sizeof(double) * c->N, host, w, w, nfields,
+                               hipMemcpyHostToDevice, c->s));
     return 0;
 }
 
-int down(st_ctx *c, double *host, const double *dev, long nfields)
+int down(st_ctx *c, double *host, const double *dev, long nfields, int slot = 0)
 {
-    HIPCHK(c, hipMemcpyAsync(host, dev, sizeof(double) * nfields * c->N, hipMemcpyDeviceToHost, c->s));
+    const size_t w = sizeof(double) * c->ni;
+    HIPCHK(c, hipMemcpy2DAsync(host, w, dev + (size_t)slot * c->ni, sizeof(double) * c->N, w, nfields,
+                               hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (c->prof) prof_collect(c);
     return 0;
 }
 
-int fill(st_ctx *c, double *p, long n, double v)
+// fields [p, p + nfields*N) of the launch's slots (all, or those with act)
+int fill(st_ctx *c, double *p, int nfields, const st::PassArgs &base, double v)
 {
     Event e;
     if (c->prof) prof_begin(c, KIND_FILL * 3, e);
-    hipError_t err = st::run_fill(p, n, v, c->s);
+    hipError_t err = st::run_fill_slots(p, nfields, c->N, c->ni, base.nslot, base.act, v, c->s);
     if (c->prof) prof_end(c, e);
     if (err != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(err));
     return 0;
@@ -463,10 +507,10 @@ int fill(st_ctx *c, double *p, long n, double v)
 
 // RHS (src/rhsv_periodic_sf.f:9-25) into dQ; with the artificial transport
 // on, also the strain magnitude, sensor and first Laplacians it needs.
-int seq_rhs(st_ctx *c)
+int seq_rhs(st_ctx *c, const st::PassArgs &base)
 {
     const int hyp = c->p.hyper_param;
-    st::PassArgs a = base_args(c);
+    st::PassArgs a = base;
     a.first = 1;
     TRY(pass(c, st::K_GRAD, 1, a));
     // the z and x passes write their own partial sums (no read-modify-write);
@@ -505,10 +549,10 @@ int seq_rhs(st_ctx *c)
 }
 
 // Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ.
-int seq_hyper(st_ctx *c)
+int seq_hyper(st_ctx *c, const st::PassArgs &base)
 {
     if (!c->p.hyper_param) return 0;
-    st::PassArgs a = base_args(c);
+    st::PassArgs a = base;
     a.first = 1;
     TRY(pass(c, st::K_LAP2, 0, a));
     a.first = 0;
@@ -524,11 +568,11 @@ int seq_hyper(st_ctx *c)
 }
 
 // One stage body (src/lddrk4_periodic_sf.f:344-356).
-int seq_stage(st_ctx *c, double ra, double rb, double dt)
+int seq_stage(st_ctx *c, const st::PassArgs &base, double ra, double rb, double dt)
 {
-    TRY(seq_rhs(c));
-    TRY(seq_hyper(c));
-    st::PassArgs a = base_args(c);
+    TRY(seq_rhs(c, base));
+    TRY(seq_hyper(c, base));
+    st::PassArgs a = base;
     a.rk_a = ra;
     a.rk_b = rb;
     a.dt = dt;
@@ -566,15 +610,49 @@ int reduce(st_ctx *c, int which, int nv, double *out)
 
 int stage_loop(st_ctx *c, double dt, double *time, int *step)
 {
+    const st::PassArgs base = base_args(c);
     for (int loop = 0; loop < 2; loop++) {
         const double *ra, *rb;
         const int ns = rk_loop(c->p.rk_scheme, loop, &ra, &rb);
         if (!ns) break;
-        TRY(fill(c, c->QW, 5 * c->N, 0.0));                  // QW = 0 (:339, :376)
-        for (int k = 0; k < ns; k++) TRY(seq_stage(c, ra[k], rb[k], dt));
+        TRY(fill(c, c->QW, 5, base, 0.0));                   // QW = 0 (:339, :376)
+        for (int k = 0; k < ns; k++) TRY(seq_stage(c, base, ra[k], rb[k], dt));
         if (time) *time = *time + dt;                        // :372, :419
     }
     if (step) *step = *step + 1;                             // :420
+    return 0;
+}
+
+// One step of every running slot of a batch, entirely on the device: CFL
+// and dt-limit partials, dt per slot (k_dt), then the two RK loops with
+// each slot's own dt and 1/Re, and the slot clocks.  No host round trip,
+// so a step can be captured in a graph and replayed.
+int batch_step(st_ctx *c)
+{
+    st::PassArgs a = base_args(c);
+    a.nslot = c->ns;
+    a.act = c->s_act;
+    int nb = 0;
+    double *pc = c->red, *pt = c->red + (long)st::reduce_blocks(c->ni) * c->ns * 2;
+    hipError_t err = st::run_reduce(0, a, pc, &nb, c->s);
+    if (err == hipSuccess && c->p.hyper_param == 1) err = st::run_reduce(1, a, pt, &nb, c->s);
+    if (err == hipSuccess)
+        err = st::run_dt(pc, pt, nb, c->ns, c->s_act, c->s_target, c->s_step, c->p.cfl_cal, c->p.hyper_param,
+                         c->g.nonperiodic_x ? 1 : 0, c->s_run, c->s_dt, c->s_dtlast, c->s);
+    if (err != hipSuccess) return fail(c, -4, "batch dt: %s", hipGetErrorString(err));
+    a.act = c->s_run;
+    a.sre = c->s_re;
+    a.sdt = c->s_dt;
+    for (int loop = 0; loop < 2; loop++) {
+        const double *ra, *rb;
+        const int ns = rk_loop(c->p.rk_scheme, loop, &ra, &rb);
+        if (!ns) break;
+        TRY(fill(c, c->QW, 5, a, 0.0));
+        for (int k = 0; k < ns; k++) TRY(seq_stage(c, a, ra[k], rb[k], 0.0));
+        const int last = loop == 1 || !rk_loop(c->p.rk_scheme, 1, &ra, &rb);
+        err = st::run_clock(c->ns, c->s_run, c->s_dt, c->s_time, c->s_step, last, c->s);
+        if (err != hipSuccess) return fail(c, -4, "batch clock: %s", hipGetErrorString(err));
+    }
     return 0;
 }
 
@@ -656,7 +734,7 @@ extern "C" int st_gauss_dir(st_ctx *c, int dir, const double *F, double *Fg)
 static int chain3(st_ctx *c, int kind, const double *F, double *F0, int nf)
 {
     for (int L = 0; L < nf; L++) {
-        TRY(up(c, c->tB, F + (size_t)L * c->N, 1));
+        TRY(up(c, c->tB, F + (size_t)L * c->ni, 1));
         st::PassArgs a = base_args(c);
         a.in0 = c->tB;
         a.out0 = c->tC;
@@ -667,7 +745,7 @@ static int chain3(st_ctx *c, int kind, const double *F, double *F0, int nf)
         a.in0 = c->tB;
         a.out0 = c->tC;
         TRY(pass(c, kind, 2, a));
-        TRY(down(c, F0 + (size_t)L * c->N, c->tC, 1));
+        TRY(down(c, F0 + (size_t)L * c->ni, c->tC, 1));
     }
     return 0;
 }
@@ -688,7 +766,7 @@ extern "C" int st_rhs(st_ctx *c, const double *Q, double *Qrhs)
 {
     if (!c) return -1;
     if (Q) TRY(up(c, c->Q, Q, 5));
-    TRY(seq_rhs(c));
+    TRY(seq_rhs(c, base_args(c)));
     if (Qrhs) return down(c, Qrhs, c->dQ, 5);
     return 0;
 }
@@ -698,7 +776,7 @@ extern "C" int st_hyper_rhs(st_ctx *c, double *Qrhs)
     if (!c) return -1;
     if (!c->p.hyper_param) return 0;                         // src/hyper_rhs.f:50-52
     if (Qrhs) TRY(up(c, c->dQ, Qrhs, 5));
-    TRY(seq_hyper(c));
+    TRY(seq_hyper(c, base_args(c)));
     if (Qrhs) return down(c, Qrhs, c->dQ, 5);
     return 0;
 }
@@ -765,8 +843,9 @@ extern "C" int st_tstep(st_ctx *c, double cfl_resv, int step, double *dt)
 extern "C" int st_rk_stage(st_ctx *c, double alpha, double beta, double dt, int reset_qw)
 {
     if (!c) return -1;
-    if (reset_qw) TRY(fill(c, c->QW, 5 * c->N, 0.0));
-    TRY(seq_stage(c, alpha, beta, dt));
+    const st::PassArgs base = base_args(c);
+    if (reset_qw) TRY(fill(c, c->QW, 5, base, 0.0));
+    TRY(seq_stage(c, base, alpha, beta, dt));
     return 0;
 }
 
@@ -794,6 +873,112 @@ extern "C" int st_march(st_ctx *c, int nsteps, double *time, int *step, double *
     if (c->prof) prof_collect(c);
     return 0;
 }
+
+// ---------------------------------------------------------------------
+// batches: same-size instances advanced together (no reference counterpart)
+// ---------------------------------------------------------------------
+namespace {
+
+int slot_ok(st_ctx *c, int slot)
+{
+    if (slot < 0 || slot >= c->ns) return fail(c, -1, "slot %d out of range (batch of %d)", slot, c->ns);
+    return 0;
+}
+
+int push_act(st_ctx *c)
+{
+    HIPCHK(c, hipMemcpy(c->s_act, c->act_h.data(), sizeof(int) * c->ns, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Capture one batch step into a graph (once); the slot tables it reads
+// live in device memory, so the graph stays valid as slots come and go.
+int batch_graph(st_ctx *c)
+{
+    if (c->gexec) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed));
+    const int r = batch_step(c);
+    const hipError_t e = hipStreamEndCapture(c->s, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, -3, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, -3, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int st_slot_load(st_ctx *c, int slot, const double *Q, double Re, double time, int step, int nsteps)
+{
+    if (!c || !Q || nsteps < 0) return fail(c, -1, "st_slot_load: bad arguments");
+    TRY(slot_ok(c, slot));
+    TRY(up(c, c->Q, Q, 5, slot));
+    // the slot's clock and target, 1/Re (src/define_periodic_sf.f:179-189),
+    // and zero artificial-transport coefficients (no Compute_hyper yet)
+    const double re_inv = Re > 0.0 ? 1.0 / Re : 0.0;
+    const int target = step + nsteps;
+    HIPCHK(c, hipMemcpyAsync(c->s_re + slot, &re_inv, sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->s_time + slot, &time, sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->s_step + slot, &step, sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->s_target + slot, &target, sizeof(int), hipMemcpyHostToDevice, c->s));
+    for (int f = 0; f < 3; f++) {
+        hipError_t e = st::run_fill(c->Hc + f * c->N + (long)slot * c->ni, c->ni, 0.0, c->s);
+        if (e != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(e));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    c->act_h[slot] = 1;
+    return push_act(c);
+}
+
+extern "C" int st_slot_store(st_ctx *c, int slot, double *Q)
+{
+    if (!c || !Q) return fail(c, -1, "st_slot_store: bad arguments");
+    TRY(slot_ok(c, slot));
+    return down(c, Q, c->Q, 5, slot);
+}
+
+extern "C" int st_slot_release(st_ctx *c, int slot)
+{
+    if (!c) return -1;
+    TRY(slot_ok(c, slot));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    c->act_h[slot] = 0;
+    return push_act(c);
+}
+
+extern "C" int st_batch_march(st_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return -1;
+    if (c->prof || !c->use_graph || getenv("ST_NO_GRAPH")) {
+        for (int i = 0; i < nsteps; i++) TRY(batch_step(c));
+        return 0;
+    }
+    TRY(batch_graph(c));
+    for (int i = 0; i < nsteps; i++) HIPCHK(c, hipGraphLaunch(c->gexec, c->s));
+    return 0;
+}
+
+extern "C" int st_batch_status(st_ctx *c, int *step, double *time, double *dt_last, int *active)
+{
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    const size_t ns = (size_t)c->ns;
+    if (step) HIPCHK(c, hipMemcpy(step, c->s_step, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    if (time) HIPCHK(c, hipMemcpy(time, c->s_time, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    if (dt_last) HIPCHK(c, hipMemcpy(dt_last, c->s_dtlast, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    if (active) std::memcpy(active, c->act_h.data(), sizeof(int) * ns);
+    return 0;
+}
+
+extern "C" int st_batch_slots(const st_ctx *c) { return c ? c->ns : 0; }
 
 extern "C" int st_diagnostics(st_ctx *c, double *out7)
 {

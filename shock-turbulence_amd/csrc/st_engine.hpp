@@ -30,8 +30,8 @@ struct Geo {
     int dir;               // 0: x (stride My*Mz), 1: y (stride 1), 2: z (stride My)
     int n, P, B;           // line length, lanes per line, lines per tile
     int My, Mz, Mx;
-    long ntiles;
-    long npts;
+    long ntiles;           // over all slots of the launch
+    long tpi;              // tiles per instance (slot)
 };
 
 // Tile for this workgroup.  Workgroups are dispatched round-robin over the
@@ -75,12 +75,13 @@ struct Walk {
     long step;            // global index step per r
     int b, s0;            // x and z: the thread's line and first point
 
-    DEV Walk(const Geo &g, long tile)
+    // tile: the tile within its instance; off: the instance's first point
+    DEV Walk(const Geo &g, long tile, long off)
     {
         const int t = threadIdx.x;
         const long line0 = tile * B;
         if (g.dir == 1) {
-            base = line0 * n + t;
+            base = off + line0 * n + t;
             step = NT;
             b = 0;
             s0 = 0;
@@ -90,12 +91,12 @@ struct Walk {
         s0 = t / B;
         if (g.dir == 0) {
             const long sx = (long)g.My * g.Mz;
-            base = line0 + b + s0 * sx;
+            base = off + line0 + b + s0 * sx;
             step = P * sx;
         } else {
             const unsigned line = (unsigned)(line0 + b);
             const unsigned i = line / (unsigned)g.My, j = line - i * (unsigned)g.My;
-            base = j + (long)g.My * g.Mz * i + (long)s0 * g.My;
+            base = off + j + (long)g.My * g.Mz * i + (long)s0 * g.My;
             step = (long)P * g.My;
         }
     }

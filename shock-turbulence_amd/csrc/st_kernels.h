@@ -26,9 +26,23 @@ struct PassArgs {
     int iadd;             // single-operator accumulate flag
     int first;            // accumulators: 1 = overwrite
     int fix;              // z filter: apply the pressure fix
-    int pad_;
+    int nslot;            // instances (slots) in the launch
     double rk_a, rk_b, dt;
+    // A batch stacks same-size instances field by field: point i of slot s
+    // is element s*ni + i of each field of N = nslot*ni points.
+    long ni;              // points per instance
+    const int *act;       // per-slot flag (device): 0 skips the slot; null: all
+    const double *sre;    // per-slot 1/Re (device); null: ph.re_inv
+    const double *sdt;    // per-slot dt (device); null: dt
 };
+
+// The parameters of slot `slot` (its 1/Re and dt) over the launch-wide ones.
+__device__ __forceinline__ void slot_params(const PassArgs &a, int slot, PassArgs &out)
+{
+    out = a;
+    if (a.sre) out.ph.re_inv = a.sre[slot];
+    if (a.sdt) out.dt = a.sdt[slot];
+}
 
 enum PassKind {
     K_D1, K_D2, K_FILT, K_GAUSS,
@@ -55,8 +69,15 @@ hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s);
 }  // namespace st
 
 namespace st {
-// which: 0 = CFL (2 maxima), 1 = dt limits (3 minima), 2 = diagnostics (7 sums)
+// which: 0 = CFL (2 maxima), 1 = dt limits (3 minima), 2 = diagnostics (7 sums);
+// partials per slot and block: partial[(slot*nblocks + block)*NV + v]
 hipError_t run_reduce(int which, const PassArgs &a, double *partial, int *nblocks, hipStream_t s);
+int reduce_blocks(long ni);
 hipError_t run_fill(double *p, long n, double v, hipStream_t s);
+hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *act, double v, hipStream_t s);
 hipError_t run_fixp(const PassArgs &a, double *Qx, hipStream_t s);
+hipError_t run_dt(const double *pc, const double *pt, int nb, int nslot, const int *active, const int *target,
+                  const int *stepc, double cfl_cal, int hyper, int first_step, int *run, double *dt, double *dt_last,
+                  hipStream_t s);
+hipError_t run_clock(int nslot, const int *run, const double *dt, double *time, int *stepc, int bump, hipStream_t s);
 }  // namespace st

@@ -23,10 +23,15 @@ namespace st {
 // second derivatives, and :1252-1275), plus |S| (src/hyper_dynamic.f:12-33,
 // 227-245) and the dilatation sensor C_beta (src/hyper_dynamic.f:142-160;
 // skipped when Cb is null: the constant-coefficient src/hyper_routines.f).
-__global__ void __launch_bounds__(256) k_cross(PassArgs a, int hyp)
+__global__ void __launch_bounds__(256) k_cross(PassArgs a0, int hyp)
 {
-    const long N = a.N;
-    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < N; gi += (long)gridDim.x * blockDim.x) {
+    const int slot = blockIdx.y;
+    if (a0.act && !a0.act[slot]) return;
+    PassArgs a;
+    slot_params(a0, slot, a);
+    const long N = a.N, off = (long)slot * a.ni;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
+        const long gi = off + i;
         const Phys &ph = a.ph;
         const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
         const Prim pr = prim(ph, q1, q2, q3, q4, q5);
@@ -118,17 +123,16 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
 
 hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s)
 {
-    const long N = a.N;
-    long blocks = (N + 255) / 256;
+    long blocks = (a.ni + 255) / 256;
     if (blocks > 2048 * 8) blocks = 2048 * 8;
-    hipLaunchKernelGGL(k_cross, dim3((unsigned)blocks), dim3(256), 0, s, a, hyp);
+    hipLaunchKernelGGL(k_cross, dim3((unsigned)blocks, (unsigned)a.nslot), dim3(256), 0, s, a, hyp);
     return hipGetLastError();
 }
 
 }  // namespace st
 
 // ---------------------------------------------------------------------
-// reductions (one partial per block; the host folds the partials)
+// reductions: one partial per block and slot, partial[(slot*nb + block)*NV + v]
 // ---------------------------------------------------------------------
 namespace st {
 
@@ -147,7 +151,7 @@ DEV void block_reduce(double (&v)[NV], OP op, double *partial)
     if (threadIdx.x < NV) {
         double x = sh[threadIdx.x][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = op(x, sh[threadIdx.x][w]);
-        partial[blockIdx.x * NV + threadIdx.x] = x;
+        partial[((long)blockIdx.y * gridDim.x + blockIdx.x) * NV + threadIdx.x] = x;
     }
 }
 
@@ -158,11 +162,14 @@ struct OpSum { DEV double operator()(double a, double b) const { return a + b; }
 // Compute_CFL src/lddrk4_periodic_sf.f:29-56
 __global__ void __launch_bounds__(256) k_cfl(PassArgs a, double *partial)
 {
-    const long N = a.N;
+    const int slot = blockIdx.y;
+    if (a.act && !a.act[slot]) return;
+    const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
     const double fac = ph.gamma * ph.gm1;
     double v[2] = {-INFINITY, -INFINITY};
-    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < N; gi += (long)gridDim.x * blockDim.x) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
+        const long gi = off + i;
         const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
         const double t1 = 1.0 / q4;
         const double t2 = sqrt(fac * t1 * (q5 - 0.5 * t1 * (q1 * q1 + q2 * q2 + q3 * q3)));
@@ -178,12 +185,15 @@ __global__ void __launch_bounds__(256) k_cfl(PassArgs a, double *partial)
 // Compute_Tstep src/lddrk4_periodic_sf.f:105-132 (beta*, mu*, kappa* in Hc)
 __global__ void __launch_bounds__(256) k_tstep(PassArgs a, double *partial)
 {
-    const long N = a.N;
+    const int slot = blockIdx.y;
+    if (a.act && !a.act[slot]) return;
+    const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
     const double fac = ph.gamma * ph.gm1;
     const double del = ph.D[0];
     double v[3] = {INFINITY, INFINITY, INFINITY};
-    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < N; gi += (long)gridDim.x * blockDim.x) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
+        const long gi = off + i;
         const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
         const double t1 = 1.0 / q4;
         const double br = q5 - 0.5 * t1 * (q1 * q1 + q2 * q2 + q3 * q3);
@@ -202,10 +212,13 @@ __global__ void __launch_bounds__(256) k_tstep(PassArgs a, double *partial)
 // Compute_Mom/KE/Mass/Energy/Entropy src/lddrk4_periodic_sf.f:153-302 (sums)
 __global__ void __launch_bounds__(256) k_diag(PassArgs a, double *partial)
 {
-    const long N = a.N;
+    const int slot = blockIdx.y;
+    if (a.act && !a.act[slot]) return;
+    const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < N; gi += (long)gridDim.x * blockDim.x) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
+        const long gi = off + i;
         const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
         const double t = 1.0 / q4;
         const double u1 = q1 * t, u2 = q2 * t, u3 = q3 * t;
@@ -226,12 +239,21 @@ __global__ void __launch_bounds__(256) k_fill(double *p, long n, double v)
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Fix_Pressure on a resident array (src/lddrk4_periodic_sf.f:436-473)
+// p[f*N + s*ni + i] = v for the active slots s (blockIdx.y = f*nslot + s)
+__global__ void __launch_bounds__(256) k_fill_slots(double *p, long N, long ni, int nslot, const int *act, double v)
+{
+    const int f = blockIdx.y / nslot, slot = blockIdx.y - f * nslot;
+    if (act && !act[slot]) return;
+    double *q = p + f * N + (long)slot * ni;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) q[i] = v;
+}
+
+// Fix_Pressure on a resident array (src/lddrk4_periodic_sf.f:436-473), slot 0
 __global__ void __launch_bounds__(256) k_fixp(PassArgs a, double *Qx)
 {
     const long N = a.N;
     const Phys &ph = a.ph;
-    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < N; gi += (long)gridDim.x * blockDim.x) {
+    for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < a.ni; gi += (long)gridDim.x * blockDim.x) {
         const double q1 = Qx[gi], q2 = Qx[N + gi], q3 = Qx[2 * N + gi], q4 = Qx[3 * N + gi], q5 = Qx[4 * N + gi];
         const double vs = 1.0 / q4;
         const double ke = q1 * q1 + q2 * q2 + q3 * q3;
@@ -239,6 +261,53 @@ __global__ void __launch_bounds__(256) k_fixp(PassArgs a, double *Qx)
         if (p < 0.0) p = 0.71463;
         Qx[4 * N + gi] = ph.gm1_inv * p + 0.5 * vs * ke;
     }
+}
+
+// Device-side Compute_Tstep for every slot (one thread per slot): fold the
+// CFL and dt-limit partials, dt = cfl_cal/cfl_resv, then the hyper limits
+// when the slot's step passes the driver's rule (src/lddrk4_periodic_sf.f:96-142;
+// first_step = 1 applies them from the first step,
+// src/lddrk4_nonperiodic_sf.f:105).  stepc is the driver's step counter
+// (1 before the first step).  A slot runs this step (run = 1) while it is
+// active and its counter is below its target.
+__global__ void k_dt(const double *pc, const double *pt, int nb, int nslot, const int *active, const int *target,
+                     const int *stepc, double cfl_cal, int hyper, int first_step, int *run, double *dt, double *dt_last)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslot) return;
+    const int go = active[slot] && stepc[slot] < target[slot];
+    run[slot] = go;
+    if (!go) return;
+    double resv = -INFINITY;
+    for (int b = 0; b < nb; b++) {
+        const double x = pc[((long)slot * nb + b) * 2 + 1];
+        resv = x > resv ? x : resv;
+    }
+    double t = cfl_cal / resv;
+    if (hyper == 1 && (stepc[slot] > 1 || first_step)) {
+        double v[3] = {INFINITY, INFINITY, INFINITY};
+        for (int b = 0; b < nb; b++)
+            for (int i = 0; i < 3; i++) {
+                const double x = pt[((long)slot * nb + b) * 3 + i];
+                v[i] = x < v[i] ? x : v[i];
+            }
+        const double p2 = (double)0.2f;                 // default-real literal 0.2
+        if (p2 * v[0] < t) t = p2 * v[0];
+        if (p2 * v[1] < t) t = p2 * v[1];
+        if (p2 * v[2] < t) t = p2 * v[2];
+    }
+    dt[slot] = t;
+    dt_last[slot] = t;
+}
+
+// time += dt after each RK loop; step += 1 after the second
+// (src/lddrk4_periodic_sf.f:372,419-420)
+__global__ void k_clock(int nslot, const int *run, const double *dt, double *time, int *stepc, int bump)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslot || !run[slot]) return;
+    time[slot] = time[slot] + dt[slot];
+    if (bump) stepc[slot] = stepc[slot] + 1;
 }
 
 static long red_blocks(long N)
@@ -249,13 +318,16 @@ static long red_blocks(long N)
 
 hipError_t run_reduce(int which, const PassArgs &a, double *partial, int *nblocks, hipStream_t s)
 {
-    const long nb = red_blocks(a.N);
+    const long nb = red_blocks(a.ni);
     *nblocks = (int)nb;
-    if (which == 0) hipLaunchKernelGGL(k_cfl, dim3((unsigned)nb), dim3(256), 0, s, a, partial);
-    else if (which == 1) hipLaunchKernelGGL(k_tstep, dim3((unsigned)nb), dim3(256), 0, s, a, partial);
-    else hipLaunchKernelGGL(k_diag, dim3((unsigned)nb), dim3(256), 0, s, a, partial);
+    const dim3 grid((unsigned)nb, (unsigned)a.nslot);
+    if (which == 0) hipLaunchKernelGGL(k_cfl, grid, dim3(256), 0, s, a, partial);
+    else if (which == 1) hipLaunchKernelGGL(k_tstep, grid, dim3(256), 0, s, a, partial);
+    else hipLaunchKernelGGL(k_diag, grid, dim3(256), 0, s, a, partial);
     return hipGetLastError();
 }
+
+int reduce_blocks(long ni) { return (int)red_blocks(ni); }
 
 hipError_t run_fill(double *p, long n, double v, hipStream_t s)
 {
@@ -265,10 +337,34 @@ hipError_t run_fill(double *p, long n, double v, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *act, double v, hipStream_t s)
+{
+    long nb = (ni + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_fill_slots, dim3((unsigned)nb, (unsigned)(nfields * nslot)), dim3(256), 0, s, p, N, ni, nslot,
+                       act, v);
+    return hipGetLastError();
+}
+
 hipError_t run_fixp(const PassArgs &a, double *Qx, hipStream_t s)
 {
-    const long nb = red_blocks(a.N) * 4;
+    const long nb = red_blocks(a.ni) * 4;
     hipLaunchKernelGGL(k_fixp, dim3((unsigned)nb), dim3(256), 0, s, a, Qx);
+    return hipGetLastError();
+}
+
+hipError_t run_dt(const double *pc, const double *pt, int nb, int nslot, const int *active, const int *target,
+                  const int *stepc, double cfl_cal, int hyper, int first_step, int *run, double *dt, double *dt_last,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt, dim3((unsigned)((nslot + 63) / 64)), dim3(64), 0, s, pc, pt, nb, nslot, active, target,
+                       stepc, cfl_cal, hyper, first_step, run, dt, dt_last);
+    return hipGetLastError();
+}
+
+hipError_t run_clock(int nslot, const int *run, const double *dt, double *time, int *stepc, int bump, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clock, dim3((unsigned)((nslot + 63) / 64)), dim3(64), 0, s, nslot, run, dt, time, stepc, bump);
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // reach store_early() (REUSE) or store() appended after the results.
 template <int M, int P, int NT, bool NP, class PASS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
-k_pass(Geo g, PassArgs a)
+k_pass(Geo g, PassArgs a0)
 {
     extern __shared__ double lds[];
     constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
@@ -40,7 +40,12 @@ k_pass(Geo g, PassArgs a)
     constexpr int NPF = PASS::NPF;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
-    const Walk<M, P, NT> W(g, tile_of(g));
+    const long tile = tile_of(g);
+    const int slot = (int)(tile / g.tpi);
+    if (a0.act && !a0.act[slot]) return;          // uniform per workgroup, before any barrier
+    PassArgs a;
+    slot_params(a0, slot, a);
+    const Walk<M, P, NT> W(g, tile - (long)slot * g.tpi, (long)slot * a.ni);
     constexpr int ss = NT * M;           // slot size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
@@ -210,13 +215,13 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
     g.My = My;
     g.Mz = Mz;
     g.Mx = Mx;
-    g.npts = (long)My * Mz * Mx;
     g.P = t.P;
     g.B = nt / t.P;
     if (g.B < 1 || g.B * t.P != nt) return hipErrorInvalidValue;
-    const long lot = g.npts / g.n;
-    if (lot % g.B) return hipErrorInvalidValue;
-    g.ntiles = lot / g.B;
+    const long lot = (long)My * Mz * Mx / g.n;          // lines per instance
+    if (lot % g.B || a.ni != (long)My * Mz * Mx || a.nslot < 1) return hipErrorInvalidValue;
+    g.tpi = lot / g.B;
+    g.ntiles = g.tpi * a.nslot;
     // non-periodic lines are the x lines of the shock-tube link set
     if (t.nonper && dir == 0) {
         if constexpr (XL) {

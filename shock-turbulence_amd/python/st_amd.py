@@ -45,7 +45,8 @@ EXPORTS = [
     "st_state_dev", "st_d1", "st_d2", "st_filter", "st_filter_dir", "st_gauss", "st_gauss_dir", "st_rhs",
     "st_hyper_rhs", "st_get_hyper", "st_set_hyper", "st_get_grad", "st_fix_pressure", "st_cfl", "st_tstep",
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
-    "st_ic_isotropic",
+    "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
+    "st_batch_march", "st_batch_status",
 ]
 
 _lib = None
@@ -94,6 +95,13 @@ def lib():
     L.st_pass_name.restype = ctypes.c_char_p
     L.st_ic_isotropic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, _dp]
+    L.st_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.st_batch_slots.argtypes = [vp]
+    L.st_slot_load.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.st_slot_store.argtypes = [vp, ctypes.c_int, _dp]
+    L.st_slot_release.argtypes = [vp, ctypes.c_int]
+    L.st_batch_march.argtypes = [vp, ctypes.c_int]
+    L.st_batch_status.argtypes = [vp, ip, _dp, _dp, ip]
     _lib = L
     return L
 
@@ -133,17 +141,19 @@ def re_for_taylor(re_lambda, k0, urms, mt, gamma=1.4, vis_exp=0.76):
 
 
 class Solver:
-    """One device context (grid + parameters + resident state)."""
+    """One device context (grid + parameters + resident state); nslots > 1
+    makes a batch (see Batch)."""
 
     def __init__(self, nx, ny, nz, period=(2 * np.pi,) * 3, Re=100.0, Pr=0.7, gamma=1.4, vis_exp=0.76,
                  bulk_ratio=0.0, hyper_param=1, cfl_cal=0.5, d2_mode=0, rk_scheme=0, start=(0.0, 0.0, 0.0),
-                 nonperiodic_x=0):
+                 nonperiodic_x=0, nslots=1):
         self.grid = Grid(nx, ny, nz, period[0], period[1], period[2], start[0], start[1], start[2],
                          1 if nonperiodic_x else 0)
         self.params = Params(Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal, d2_mode, rk_scheme)
         self.shape = (ny, nz, nx)
+        self.nslots = nslots
         h = ctypes.c_void_p()
-        r = lib().st_ctx_create(ctypes.byref(self.grid), ctypes.byref(self.params), ctypes.byref(h))
+        r = lib().st_batch_create(ctypes.byref(self.grid), ctypes.byref(self.params), nslots, ctypes.byref(h))
         self.h = h
         if r:
             msg = lib().st_last_error(h).decode() if h.value else "context creation failed"
@@ -298,3 +308,38 @@ class Solver:
             if cnt[i]:
                 out[lib().st_pass_name(i).decode()] = {"count": cnt[i], "ms": ms[i], "fields": fr[i]}
         return out
+
+
+class Batch(Solver):
+    """A batch of same-size instances advanced together (st_batch_*)."""
+
+    def __init__(self, nx, ny, nz, nslots, **kw):
+        super().__init__(nx, ny, nz, nslots=nslots, **kw)
+
+    def load(self, slot, Q, Re, nsteps, time=0.0, step=0):
+        Q = _f(Q)
+        self._chk(lib().st_slot_load(self.h, slot, _p(Q), Re, time, step, nsteps), "st_slot_load")
+
+    def store(self, slot):
+        Q = self._z(5)
+        self._chk(lib().st_slot_store(self.h, slot, _p(Q)), "st_slot_store")
+        return Q
+
+    def release(self, slot):
+        self._chk(lib().st_slot_release(self.h, slot), "st_slot_release")
+
+    def march(self, nsteps):
+        """Enqueue nsteps batch steps (asynchronous; status() or sync() waits)."""
+        self._chk(lib().st_batch_march(self.h, nsteps), "st_batch_march")
+
+    def sync(self):
+        self._chk(lib().st_sync(self.h), "st_sync")
+
+    def status(self):
+        n = self.nslots
+        step = (ctypes.c_int * n)()
+        act = (ctypes.c_int * n)()
+        t = np.zeros(n)
+        dt = np.zeros(n)
+        self._chk(lib().st_batch_status(self.h, step, _p(t), _p(dt), act), "st_batch_status")
+        return np.array(step[:]), t, dt, np.array(act[:])
