@@ -8,6 +8,12 @@ LDDRK4 = 5 + 6 RK stages (RHS, Hyper_RHS, update, compact filter,
 pressure fix).  The metric is grid points advanced through one RK stage
 per second (Mpts*stage/s), with the state resident in HBM.
 
+Beside the headline it runs two bounded legs: a batched ensemble of 48
+instances of 48^3..128^3 against the same instances one at a time, and a
+finite Poisson trace of runs served with continuous batching against the
+plain path one at a time (p50/p99 time to completion, throughput, and a
+check that every run's result is the same bits alone and in the mix).
+
 Prints one JSON line.  With --gpus N > 1 it exits: this build measures a
 single GPU.
 """
@@ -70,6 +76,60 @@ def cpu_baseline(n, threads, Q, kw):
                       "same initial state, oracle/st_oracle.c with %d OpenMP threads, %.2f s" % (n, threads, el)}
 
 
+def ensemble_leg(nsteps, autotune):
+    """Batched ensemble (48 instances: 16 x 48^3, 16 x 64^3, 12 x 96^3, 4 x
+    128^3) against the same instances one at a time on the plain path."""
+    import st_amd
+    import st_serve
+    runs = st_serve.ensemble_config3(nsteps)
+    for r in runs:
+        r.make_ic()
+    work = sum(r.n ** 3 * STAGES_PER_STEP * nsteps for r in runs)
+    ens = st_serve.Ensemble(runs, autotune=autotune)
+    ens.march(0)
+    t0 = time.perf_counter()
+    ens.march(nsteps)
+    el = time.perf_counter() - t0
+    ens.close()
+    plain = 0.0
+    for r in runs:
+        s = st_amd.Solver(r.n, r.n, r.n, Re=r.Re, Pr=0.7, hyper_param=1, cfl_cal=0.5)
+        s.upload(r.Q)
+        s.sync()
+        t0 = time.perf_counter()
+        s.march(nsteps)
+        s.sync()
+        plain += time.perf_counter() - t0
+        s.close()
+    return {"workload": "48 instances: 16x48^3, 16x64^3, 12x96^3, 4x128^3, seeds 1..48, k0 in {3,4,6}, "
+                        "Mt in {0.2,0.3,0.5}, Re_lambda=40; %d steps each" % nsteps,
+            "points": sum(r.n ** 3 for r in runs), "steps": nsteps,
+            "batched_mpts_stage_per_s": round(work / el / 1e6, 1),
+            "plain_one_at_a_time_mpts_stage_per_s": round(work / plain / 1e6, 1),
+            "speedup": round(plain / el, 3), "autotune_reps": autotune}
+
+
+def serving_leg(nruns, rate, seed):
+    """A finite Poisson trace of runs (sizes 48^3..128^3, 50..400 steps)
+    served with continuous batching, against the plain path one at a time
+    (first come, first served on the same GPU)."""
+    import st_serve
+    runs = st_serve.poisson_trace(nruns, rate=rate, seed=seed)
+    srv = st_serve.Server(runs, budget_bytes=200e9)
+    try:
+        served = srv.serve()
+    finally:
+        srv.close()
+    plain, ref = st_serve.plain_one_at_a_time(runs)
+    same = all(bool((r.result == ref[r.rid]).all()) for r in runs)
+    return {"workload": "%d runs, Poisson arrivals at %.0f/s, N in {48,64,96,128}^3 uniform, steps U[50,400], "
+                        "seed %d; HBM budget 200 GB" % (nruns, rate, seed),
+            "served": {k: round(v, 4) if isinstance(v, float) else v for k, v in served.items()},
+            "plain_one_at_a_time": {k: round(v, 4) if isinstance(v, float) else v for k, v in plain.items()},
+            "throughput_ratio": round(served["mpts_stage_per_s"] / plain["mpts_stage_per_s"], 3),
+            "results_equal_alone_and_in_mix": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +140,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--profile", type=int, default=1, help="per-pass HIP-event timing (in extra steps after the timed region)")
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--ensemble-steps", type=int, default=6, help="batched-ensemble leg (0: skip)")
+    ap.add_argument("--autotune", type=int, default=0, help="autotune the ensemble batches (stages per candidate)")
+    ap.add_argument("--serve-runs", type=int, default=16, help="serving leg: runs in the trace (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 2 GiB fill and a 2 GiB copy first (reference kernels for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -175,6 +239,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_threads, Q, kw)
+    del Q, Qend
+    ensemble = ensemble_leg(args.ensemble_steps, args.autotune) if args.ensemble_steps > 0 else None
+    serving = serving_leg(args.serve_runs, args.serve_rate, 2024) if args.serve_runs > 0 else None
 
     out = {
         "metric": "Mpts*RK-stage/s FP64 (256^3 periodic RHS+LDDRK4 stage throughput)",
@@ -197,6 +264,8 @@ def main():
         "roofline_stage": stage_roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "ensemble": ensemble,
+        "serving": serving,
         "check": {"finite": finite, "sim_time": time_, "step": step, "dt_last": float(dts[-1]) if len(dts) else None,
                   "diagnostics": [float(x) for x in diag]},
     }

@@ -142,6 +142,18 @@ int st_batch_march(st_ctx *ctx, int nsteps);
 /* Per-slot step counter, time, last dt and active flag (arrays of nslots;
  * any may be NULL); waits for the stream. */
 int st_batch_status(st_ctx *ctx, int *step, double *time, double *dt_last, int *active);
+/* 1 when all work enqueued on the context's stream has finished, 0 while
+ * it runs (does not wait). */
+int st_query(st_ctx *ctx);
+
+/* Autotuner (no counterpart in the reference): time every pass of an RK
+ * stage over all slots under each lane layout (M or M/2 points per lane) and
+ * tile width (256, 512, 1024 threads) and keep the fastest per pass and
+ * direction; reps stages per candidate.  The state is restored.
+ * st_pass_config reads the choice for kind_dir = kind*3 + direction (the
+ * index of st_pass_name). */
+int st_autotune(st_ctx *ctx, int reps);
+int st_pass_config(st_ctx *ctx, int kind_dir, int *narrow, int *nt);
 
 /* Seeded isotropic-turbulence initial condition (new; the reference restarts
  * from a file, src/main.f:69-71).  Q(My,Mz,Mx,5) on the host. */

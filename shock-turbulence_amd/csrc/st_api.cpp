@@ -43,7 +43,7 @@ struct st_ctx {
     st::Phys ph;
     st::Coef C;
     st::DirTab th[6];               // [dir] M points per lane, [3+dir] M/2 points per lane
-    int cfg_narrow[NKIND], cfg_nt[NKIND][3];
+    int cfg_narrow[NKIND][3], cfg_nt[NKIND][3];
     st::DirTab *td = nullptr;
     double *buf = nullptr;
     double *Q, *QW, *dQ, *G, *L2e, *S, *Cb, *L2S, *L4e, *L4S, *eta, *Hc, *T5, *tB, *tC;
@@ -53,10 +53,11 @@ struct st_ctx {
     long red_n = 0;                 // doubles in red / red_h
     // per-slot state of a batch (device): active flag, run flag of the
     // current step, step target, driver step counter, 1/Re, dt, time
-    int *s_act = nullptr, *s_run, *s_target, *s_step;
+    int *s_act = nullptr, *s_run, *s_target, *s_step, *s_list;
     double *s_re = nullptr, *s_dt, *s_time, *s_dtlast;
     std::vector<int> act_h;
-    hipGraphExec_t gexec = nullptr; // one batch step, captured
+    int nact = 0;                   // active slots, listed in s_list (launch grids cover these only)
+    std::vector<hipGraphExec_t> graphs;   // one batch step, captured per active-slot count
     int use_graph = 1;
     hipStream_t s = nullptr;
     char err[512];
@@ -184,20 +185,22 @@ double kind_fields(int kind, const st::PassArgs &a)
 int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
 {
     const int pk = kind * 3 + dir;
-    const int lay = c->cfg_narrow[kind] ? 3 : 0;
+    const int lay = c->cfg_narrow[kind][dir] ? 3 : 0;
     const st::DirTab &t = c->th[lay + dir];
     a.tab = c->td + lay + dir;
     a.K1 = cadd * c->sx1[dir] * (double)(t.n - 1) * t.op[0].scale;   // fac of src/derivCDopt6_periodic.f:181
     a.K2 = t.s2 * t.op[1].scale;
     Event e;
-    if (c->prof) prof_begin(c, pk, e);
+    const int prof = c->prof && !a.dry;
+    if (prof) prof_begin(c, pk, e);
     // the widest configured tile that the line count and layout allow
     int nt = c->cfg_nt[kind][dir];
     const long lot = c->ni / t.n;
     // (1024-thread tiles exist for M = 4, 8 with 32 or 64 lanes per line)
     while (nt > 256 && ((nt == 1024 && (t.P < 32 || (t.M & (t.M - 1)))) || lot % (nt / t.P))) nt /= 2;
     hipError_t err = st::run_pass(kind, t, nt, dir, c->My, c->Mz, c->Mx, a, c->s);
-    if (c->prof) {
+    if (a.dry) return err == hipSuccess ? 0 : -4;
+    if (prof) {
         prof_end(c, e);
         c->prof_fields[pk] = kind_fields(kind, a);
     }
@@ -233,6 +236,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     c->ni = (long)c->Mx * c->My * c->Mz;
     c->N = c->ni * nslots;
     c->act_h.assign(nslots, 0);
+    c->graphs.assign(nslots + 1, nullptr);
     std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
     std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
     std::memset(c->prof_fields, 0, sizeof(c->prof_fields));
@@ -242,11 +246,12 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
         const char *en = getenv("ST_NARROW"), *et = getenv("ST_NT512"), *eu = getenv("ST_NT1024");
         for (int k = 0; k < NKIND; k++) {
             for (int d = 0; d < 3; d++) {
-                st::pass_default_cfg(k, d, &c->cfg_narrow[k], &c->cfg_nt[k][d]);
+                st::pass_default_cfg(k, d, &c->cfg_narrow[k][d], &c->cfg_nt[k][d]);
                 if (et) c->cfg_nt[k][d] = ((strtoul(et, nullptr, 0) >> k) & 1) ? 512 : 256;
                 if (eu && ((strtoul(eu, nullptr, 0) >> k) & 1)) c->cfg_nt[k][d] = 1024;
             }
-            if (en) c->cfg_narrow[k] = (int)((strtoul(en, nullptr, 0) >> k) & 1);
+            if (en)
+                for (int d = 0; d < 3; d++) c->cfg_narrow[k][d] = (int)((strtoul(en, nullptr, 0) >> k) & 1);
             // ST_NT_<k> = "nx,ny,nz": tile threads of pass kind k per direction
             char var[32];
             snprintf(var, sizeof(var), "ST_NT_%d", k);
@@ -362,7 +367,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
         (e = hipMalloc(&c->td, sizeof(st::DirTab) * 6)) != hipSuccess ||
         (e = hipMalloc(&c->red, sizeof(double) * c->red_n)) != hipSuccess ||
         (e = hipHostMalloc(&c->red_h, sizeof(double) * c->red_n)) != hipSuccess ||
-        (e = hipMalloc(&sp, ns * (4 * sizeof(int) + 4 * sizeof(double)))) != hipSuccess) {
+        (e = hipMalloc(&sp, ns * (6 * sizeof(int) + 4 * sizeof(double)))) != hipSuccess) {
         fail(c, -3, "device allocation: %s", hipGetErrorString(e));
         *out = c;
         return -3;
@@ -375,6 +380,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     c->s_run = c->s_act + ns;
     c->s_target = c->s_run + ns;
     c->s_step = c->s_target + ns;
+    c->s_list = c->s_step + ns;
     double *p = c->buf;
     const long N = c->N;
     c->Q = p; p += 5 * N;
@@ -396,7 +402,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     c->L2ep = p; p += 2 * N;
     if ((e = hipMemcpyAsync(c->td, c->th, sizeof(st::DirTab) * 6, hipMemcpyHostToDevice, c->s)) != hipSuccess ||
         (e = st::run_fill(c->buf, nf * N, 0.0, c->s)) != hipSuccess ||
-        (e = st::run_fill(c->s_re, 4 * (long)ns + 2 * (long)ns, 0.0, c->s)) != hipSuccess ||
+        (e = st::run_fill(c->s_re, 4 * (long)ns + 3 * (long)ns, 0.0, c->s)) != hipSuccess ||
         (e = hipStreamSynchronize(c->s)) != hipSuccess) {
         fail(c, -3, "device init: %s", hipGetErrorString(e));
         *out = c;
@@ -420,7 +426,8 @@ extern "C" void st_ctx_destroy(st_ctx *c)
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    for (auto g : c->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->s_re) (void)hipFree(c->s_re);
     if (c->buf) (void)hipFree(c->buf);
     if (c->td) (void)hipFree(c->td);
@@ -493,7 +500,7 @@ int fill(st_ctx *c, double *p, int nfields, const st::PassArgs &base, double v)
 {
     Event e;
     if (c->prof) prof_begin(c, KIND_FILL * 3, e);
-    hipError_t err = st::run_fill_slots(p, nfields, c->N, c->ni, base.nslot, base.act, v, c->s);
+    hipError_t err = st::run_fill_slots(p, nfields, c->N, c->ni, base.nslot, base.slots, base.act, v, c->s);
     if (c->prof) prof_end(c, e);
     if (err != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(err));
     return 0;
@@ -629,8 +636,10 @@ int stage_loop(st_ctx *c, double dt, double *time, int *step)
 // so a step can be captured in a graph and replayed.
 int batch_step(st_ctx *c)
 {
+    if (!c->nact) return 0;
     st::PassArgs a = base_args(c);
-    a.nslot = c->ns;
+    a.nslot = c->nact;
+    a.slots = c->s_list;
     a.act = c->s_act;
     int nb = 0;
     double *pc = c->red, *pt = c->red + (long)st::reduce_blocks(c->ni) * c->ns * 2;
@@ -885,17 +894,25 @@ int slot_ok(st_ctx *c, int slot)
     return 0;
 }
 
+// Upload the active flags and the list of active slots (the launches of a
+// batch step cover the listed slots only).
 int push_act(st_ctx *c)
 {
+    std::vector<int> list;
+    for (int i = 0; i < c->ns; i++)
+        if (c->act_h[i]) list.push_back(i);
+    c->nact = (int)list.size();
     HIPCHK(c, hipMemcpy(c->s_act, c->act_h.data(), sizeof(int) * c->ns, hipMemcpyHostToDevice));
+    if (c->nact) HIPCHK(c, hipMemcpy(c->s_list, list.data(), sizeof(int) * c->nact, hipMemcpyHostToDevice));
     return 0;
 }
 
-// Capture one batch step into a graph (once); the slot tables it reads
-// live in device memory, so the graph stays valid as slots come and go.
+// Capture one batch step into a graph, once per active-slot count (the
+// launch grids cover the active slots); the slot list and tables it reads
+// live in device memory, so a graph stays valid as slots come and go.
 int batch_graph(st_ctx *c)
 {
-    if (c->gexec) return 0;
+    if (c->graphs[c->nact]) return 0;
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed));
     const int r = batch_step(c);
@@ -905,12 +922,11 @@ int batch_graph(st_ctx *c)
         return r;
     }
     if (e != hipSuccess) return fail(c, -3, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) {
-        c->gexec = nullptr;
-        return fail(c, -3, "hipGraphInstantiate: %s", hipGetErrorString(ei));
-    }
+    if (ei != hipSuccess) return fail(c, -3, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    c->graphs[c->nact] = x;
     return 0;
 }
 
@@ -957,12 +973,13 @@ extern "C" int st_slot_release(st_ctx *c, int slot)
 extern "C" int st_batch_march(st_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return -1;
+    if (!c->nact) return 0;
     if (c->prof || !c->use_graph || getenv("ST_NO_GRAPH")) {
         for (int i = 0; i < nsteps; i++) TRY(batch_step(c));
         return 0;
     }
     TRY(batch_graph(c));
-    for (int i = 0; i < nsteps; i++) HIPCHK(c, hipGraphLaunch(c->gexec, c->s));
+    for (int i = 0; i < nsteps; i++) HIPCHK(c, hipGraphLaunch(c->graphs[c->nact], c->s));
     return 0;
 }
 
@@ -979,6 +996,109 @@ extern "C" int st_batch_status(st_ctx *c, int *step, double *time, double *dt_la
 }
 
 extern "C" int st_batch_slots(const st_ctx *c) { return c ? c->ns : 0; }
+
+extern "C" int st_query(st_ctx *c)
+{
+    if (!c) return -1;
+    const hipError_t e = hipStreamQuery(c->s);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(c, -3, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+// Pick each pass's lane layout (M or M/2 points per lane) and tile width
+// (256, 512, 1024 threads) by timing: every candidate in turn runs `reps`
+// RK stages over all slots with per-pass HIP events, and each (pass,
+// direction) keeps its fastest valid candidate.  The state and the
+// artificial-transport coefficients are restored afterwards.
+extern "C" int st_autotune(st_ctx *c, int reps)
+{
+    if (!c || reps < 1) return fail(c, -1, "st_autotune: bad arguments");
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    prof_collect(c);
+    double *bak = nullptr;
+    HIPCHK(c, hipMalloc(&bak, sizeof(double) * 8 * c->N));
+    const size_t f5 = sizeof(double) * 5 * c->N, f3 = sizeof(double) * 3 * c->N;
+    hipError_t e = hipMemcpyAsync(bak, c->Q, f5, hipMemcpyDeviceToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(bak + 5 * c->N, c->Hc, f3, hipMemcpyDeviceToDevice, c->s);
+    if (e != hipSuccess) {
+        (void)hipFree(bak);
+        return fail(c, -3, "autotune backup: %s", hipGetErrorString(e));
+    }
+    int dn[NKIND][3], dt[NKIND][3], bn[NKIND][3], bt[NKIND][3];
+    double best[NKIND][3];
+    std::memcpy(dn, c->cfg_narrow, sizeof(dn));
+    std::memcpy(dt, c->cfg_nt, sizeof(dt));
+    std::memcpy(bn, dn, sizeof(bn));
+    std::memcpy(bt, dt, sizeof(bt));
+    for (int k = 0; k < NKIND; k++)
+        for (int d = 0; d < 3; d++) best[k][d] = INFINITY;
+    static const int cand[6][2] = {{0, 256}, {0, 512}, {0, 1024}, {1, 256}, {1, 512}, {1, 1024}};
+    st::PassArgs base = base_args(c);
+    base.nslot = c->ns;
+    st::PassArgs dry = base;
+    dry.dry = 1;
+    const double *ra, *rb;
+    rk_loop(c->p.rk_scheme, 0, &ra, &rb);
+    const int prof_was = c->prof;
+    int rc = 0;
+    for (int ci = 0; ci < 6 && !rc; ci++) {
+        int valid[NKIND][3] = {};
+        for (int k = st::K_GRAD; k <= st::K_F_Z; k++)
+            for (int d = 0; d < 3; d++) {
+                c->cfg_narrow[k][d] = cand[ci][0];
+                c->cfg_nt[k][d] = cand[ci][1];
+                valid[k][d] = pass(c, k, d, dry) == 0;
+                if (!valid[k][d]) {
+                    c->cfg_narrow[k][d] = dn[k][d];
+                    c->cfg_nt[k][d] = dt[k][d];
+                }
+            }
+        c->err[0] = 0;
+        c->prof = 1;
+        std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
+        std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
+        for (int r = 0; r < reps && !rc; r++) rc = seq_stage(c, base, ra[0], rb[0], 1.0e-6);
+        if (hipStreamSynchronize(c->s) != hipSuccess && !rc) rc = fail(c, -3, "autotune: stream error");
+        prof_collect(c);
+        for (int k = st::K_GRAD; k <= st::K_F_Z; k++)
+            for (int d = 0; d < 3; d++) {
+                const int pk = k * 3 + d;
+                if (!valid[k][d] || !c->prof_cnt[pk]) continue;
+                const double ms = c->prof_ms[pk] / (double)c->prof_cnt[pk];
+                if (ms < best[k][d]) {
+                    best[k][d] = ms;
+                    bn[k][d] = cand[ci][0];
+                    bt[k][d] = cand[ci][1];
+                }
+            }
+    }
+    std::memcpy(c->cfg_narrow, rc ? dn : bn, sizeof(bn));
+    std::memcpy(c->cfg_nt, rc ? dt : bt, sizeof(bt));
+    c->prof = prof_was;
+    std::memset(c->prof_ms, 0, sizeof(c->prof_ms));
+    std::memset(c->prof_cnt, 0, sizeof(c->prof_cnt));
+    e = hipMemcpyAsync(c->Q, bak, f5, hipMemcpyDeviceToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->Hc, bak + 5 * c->N, f3, hipMemcpyDeviceToDevice, c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    (void)hipFree(bak);
+    for (auto &g : c->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, -3, "autotune restore: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int st_pass_config(st_ctx *c, int kind_dir, int *narrow, int *nt)
+{
+    if (!c || kind_dir < 0 || kind_dir >= NKIND * 3) return -1;
+    if (narrow) *narrow = c->cfg_narrow[kind_dir / 3][kind_dir % 3];
+    if (nt) *nt = c->cfg_nt[kind_dir / 3][kind_dir % 3];
+    return 0;
+}
 
 extern "C" int st_diagnostics(st_ctx *c, double *out7)
 {

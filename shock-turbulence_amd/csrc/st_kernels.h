@@ -27,10 +27,13 @@ struct PassArgs {
     int first;            // accumulators: 1 = overwrite
     int fix;              // z filter: apply the pressure fix
     int nslot;            // instances (slots) in the launch
+    int dry;              // 1: check that the launch is valid, launch nothing
+    int pad2_;
     double rk_a, rk_b, dt;
     // A batch stacks same-size instances field by field: point i of slot s
     // is element s*ni + i of each field of N = nslot*ni points.
     long ni;              // points per instance
+    const int *slots;     // the launch's slots (device, nslot entries); null: 0..nslot-1
     const int *act;       // per-slot flag (device): 0 skips the slot; null: all
     const double *sre;    // per-slot 1/Re (device); null: ph.re_inv
     const double *sdt;    // per-slot dt (device); null: dt
@@ -74,7 +77,8 @@ namespace st {
 hipError_t run_reduce(int which, const PassArgs &a, double *partial, int *nblocks, hipStream_t s);
 int reduce_blocks(long ni);
 hipError_t run_fill(double *p, long n, double v, hipStream_t s);
-hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *act, double v, hipStream_t s);
+hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *slots, const int *act,
+                          double v, hipStream_t s);
 hipError_t run_fixp(const PassArgs &a, double *Qx, hipStream_t s);
 hipError_t run_dt(const double *pc, const double *pt, int nb, int nslot, const int *active, const int *target,
                   const int *stepc, double cfl_cal, int hyper, int first_step, int *run, double *dt, double *dt_last,

@@ -25,7 +25,7 @@ namespace st {
 // skipped when Cb is null: the constant-coefficient src/hyper_routines.f).
 __global__ void __launch_bounds__(256) k_cross(PassArgs a0, int hyp)
 {
-    const int slot = blockIdx.y;
+    const int slot = a0.slots ? a0.slots[blockIdx.y] : blockIdx.y;
     if (a0.act && !a0.act[slot]) return;
     PassArgs a;
     slot_params(a0, slot, a);
@@ -133,11 +133,12 @@ hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s)
 
 // ---------------------------------------------------------------------
 // reductions: one partial per block and slot, partial[(slot*nb + block)*NV + v]
+// (slot: the slot number, not its index in the launch)
 // ---------------------------------------------------------------------
 namespace st {
 
 template <int NV, class OP>
-DEV void block_reduce(double (&v)[NV], OP op, double *partial)
+DEV void block_reduce(double (&v)[NV], OP op, double *partial, int slot)
 {
     __shared__ double sh[NV][256 / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -151,7 +152,7 @@ DEV void block_reduce(double (&v)[NV], OP op, double *partial)
     if (threadIdx.x < NV) {
         double x = sh[threadIdx.x][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = op(x, sh[threadIdx.x][w]);
-        partial[((long)blockIdx.y * gridDim.x + blockIdx.x) * NV + threadIdx.x] = x;
+        partial[((long)slot * gridDim.x + blockIdx.x) * NV + threadIdx.x] = x;
     }
 }
 
@@ -162,7 +163,7 @@ struct OpSum { DEV double operator()(double a, double b) const { return a + b; }
 // Compute_CFL src/lddrk4_periodic_sf.f:29-56
 __global__ void __launch_bounds__(256) k_cfl(PassArgs a, double *partial)
 {
-    const int slot = blockIdx.y;
+    const int slot = a.slots ? a.slots[blockIdx.y] : blockIdx.y;
     if (a.act && !a.act[slot]) return;
     const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
@@ -179,13 +180,13 @@ __global__ void __launch_bounds__(256) k_cfl(PassArgs a, double *partial)
         v[0] = w1 > v[0] ? w1 : v[0];
         v[1] = w2 > v[1] ? w2 : v[1];
     }
-    block_reduce<2>(v, OpMax(), partial);
+    block_reduce<2>(v, OpMax(), partial, slot);
 }
 
 // Compute_Tstep src/lddrk4_periodic_sf.f:105-132 (beta*, mu*, kappa* in Hc)
 __global__ void __launch_bounds__(256) k_tstep(PassArgs a, double *partial)
 {
-    const int slot = blockIdx.y;
+    const int slot = a.slots ? a.slots[blockIdx.y] : blockIdx.y;
     if (a.act && !a.act[slot]) return;
     const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
@@ -206,13 +207,13 @@ __global__ void __launch_bounds__(256) k_tstep(PassArgs a, double *partial)
         v[1] = w4 < v[1] ? w4 : v[1];
         v[2] = w5 < v[2] ? w5 : v[2];
     }
-    block_reduce<3>(v, OpMin(), partial);
+    block_reduce<3>(v, OpMin(), partial, slot);
 }
 
 // Compute_Mom/KE/Mass/Energy/Entropy src/lddrk4_periodic_sf.f:153-302 (sums)
 __global__ void __launch_bounds__(256) k_diag(PassArgs a, double *partial)
 {
-    const int slot = blockIdx.y;
+    const int slot = a.slots ? a.slots[blockIdx.y] : blockIdx.y;
     if (a.act && !a.act[slot]) return;
     const long N = a.N, off = (long)slot * a.ni;
     const Phys &ph = a.ph;
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256) k_diag(PassArgs a, double *partial)
         const double p = ph.gm1 * (q5 - 0.5 * t * (q1 * q1 + q2 * q2 + q3 * q3));
         v[6] += q4 * log(p * pow(t, ph.gamma));
     }
-    block_reduce<7>(v, OpSum(), partial);
+    block_reduce<7>(v, OpSum(), partial, slot);
 }
 
 __global__ void __launch_bounds__(256) k_fill(double *p, long n, double v)
@@ -239,10 +240,12 @@ __global__ void __launch_bounds__(256) k_fill(double *p, long n, double v)
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// p[f*N + s*ni + i] = v for the active slots s (blockIdx.y = f*nslot + s)
-__global__ void __launch_bounds__(256) k_fill_slots(double *p, long N, long ni, int nslot, const int *act, double v)
+// p[f*N + s*ni + i] = v for the launch's slots s (blockIdx.y = f*nslot + index)
+__global__ void __launch_bounds__(256) k_fill_slots(double *p, long N, long ni, int nslot, const int *slots,
+                                                     const int *act, double v)
 {
-    const int f = blockIdx.y / nslot, slot = blockIdx.y - f * nslot;
+    const int f = blockIdx.y / nslot, si = blockIdx.y - f * nslot;
+    const int slot = slots ? slots[si] : si;
     if (act && !act[slot]) return;
     double *q = p + f * N + (long)slot * ni;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) q[i] = v;
@@ -337,12 +340,13 @@ hipError_t run_fill(double *p, long n, double v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *act, double v, hipStream_t s)
+hipError_t run_fill_slots(double *p, int nfields, long N, long ni, int nslot, const int *slots, const int *act,
+                          double v, hipStream_t s)
 {
     long nb = (ni + 255) / 256;
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(k_fill_slots, dim3((unsigned)nb, (unsigned)(nfields * nslot)), dim3(256), 0, s, p, N, ni, nslot,
-                       act, v);
+                       slots, act, v);
     return hipGetLastError();
 }
 

@@ -41,11 +41,12 @@ k_pass(Geo g, PassArgs a0)
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
     const long tile = tile_of(g);
-    const int slot = (int)(tile / g.tpi);
+    const int si = (int)(tile / g.tpi);
+    const int slot = a0.slots ? a0.slots[si] : si;
     if (a0.act && !a0.act[slot]) return;          // uniform per workgroup, before any barrier
     PassArgs a;
     slot_params(a0, slot, a);
-    const Walk<M, P, NT> W(g, tile - (long)slot * g.tpi, (long)slot * a.ni);
+    const Walk<M, P, NT> W(g, tile - (long)si * g.tpi, (long)slot * a.ni);
     constexpr int ss = NT * M;           // slot size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
@@ -151,6 +152,7 @@ static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
 {
     const size_t lds = (size_t)pass_slots<PASS>() * NT * M * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (a.dry) return hipSuccess;
     static size_t lds_set = 0;
     if (lds > 64 * 1024 && lds > lds_set) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass<M, P, NT, NP, PASS>),

@@ -46,7 +46,7 @@ EXPORTS = [
     "st_hyper_rhs", "st_get_hyper", "st_set_hyper", "st_get_grad", "st_fix_pressure", "st_cfl", "st_tstep",
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
-    "st_batch_march", "st_batch_status",
+    "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_query",
 ]
 
 _lib = None
@@ -102,6 +102,9 @@ def lib():
     L.st_slot_release.argtypes = [vp, ctypes.c_int]
     L.st_batch_march.argtypes = [vp, ctypes.c_int]
     L.st_batch_status.argtypes = [vp, ip, _dp, _dp, ip]
+    L.st_autotune.argtypes = [vp, ctypes.c_int]
+    L.st_query.argtypes = [vp]
+    L.st_pass_config.argtypes = [vp, ctypes.c_int, ip, ip]
     _lib = L
     return L
 
@@ -293,6 +296,19 @@ class Solver:
     def sync(self):
         self._chk(lib().st_sync(self.h), "st_sync")
 
+    def autotune(self, reps=2):
+        """Per-pass lane layout and tile width by timing; returns {pass: (narrow, nt)}."""
+        self._chk(lib().st_autotune(self.h, reps), "st_autotune")
+        out = {}
+        for k in range(72):
+            name = lib().st_pass_name(k).decode()
+            if not name or name.startswith("."):
+                continue
+            n, t = ctypes.c_int(), ctypes.c_int()
+            lib().st_pass_config(self.h, k, ctypes.byref(n), ctypes.byref(t))
+            out[name] = (n.value, t.value)
+        return out
+
     # profiling
     def profile(self, on=True):
         self._chk(lib().st_profile(self.h, 1 if on else 0), "st_profile")
@@ -334,6 +350,10 @@ class Batch(Solver):
 
     def sync(self):
         self._chk(lib().st_sync(self.h), "st_sync")
+
+    def idle(self):
+        """True when the enqueued steps have finished (does not wait)."""
+        return self._chk(lib().st_query(self.h), "st_query") == 1
 
     def status(self):
         n = self.nslots
