@@ -32,16 +32,16 @@ STAGES_PER_STEP = 11
 
 # kernel symbol of each pass (kind, direction) as rocprofv3 reports it
 KERNEL_SYMBOL = {
-    ("grad", "y"): "k_pass<4, 64, 256, PassGrad<3, false>>",
-    ("grad", "x"): "k_pass<4, 64, 1024, PassGrad<1, true>>",
-    ("grad", "z"): "k_pass<4, 64, 256, PassGrad<1, false>>",
-    ("hyper_div", "x"): "k_pass<4, 64, 1024, PassHyperDiv>",
-    ("hyper_div", "z"): "k_pass<4, 64, 1024, PassHyperDiv>",
-    ("hyper_div", "y"): "k_pass<4, 64, 256, PassHyperDiv>",
+    ("grad", "y"): "k_pass<4, 64, 256, false, PassGrad<3, false>>",
+    ("grad", "x"): "k_pass<4, 64, 1024, false, PassGrad<1, true>>",
+    ("grad", "z"): "k_pass<4, 64, 256, false, PassGrad<1, false>>",
+    ("hyper_div", "x"): "k_pass<4, 64, 1024, false, PassHyperDiv>",
+    ("hyper_div", "z"): "k_pass<4, 64, 1024, false, PassHyperDiv>",
+    ("hyper_div", "y"): "k_pass<4, 64, 256, false, PassHyperDiv>",
     "cross": "k_cross",
-    "rk_filter_x": "k_pass<4, 64, 1024, PassFilterQ<0>>",
-    "mixh_y": "k_pass<4, 64, 256, PassMix<1, true>>",
-    "mixh_z": "k_pass<4, 64, 512, PassMix<2, true>>",
+    "rk_filter_x": "k_pass<4, 64, 1024, false, PassFilterQ<0>>",
+    "mixh_y": "k_pass<4, 64, 256, false, PassMix<1, true>>",
+    "mixh_z": "k_pass<4, 64, 512, false, PassMix<2, true>>",
 }
 
 
