@@ -30,19 +30,32 @@ B_ALG = 2136.0          # algorithmic bytes per grid point per RK stage (FP64)
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 STAGES_PER_STEP = 11
 
-# kernel symbol of each pass (kind, direction) as rocprofv3 reports it
-KERNEL_SYMBOL = {
-    ("grad", "y"): "k_pass<4, 64, 256, false, PassGrad<3, false>>",
-    ("grad", "x"): "k_pass<4, 64, 1024, false, PassGrad<1, true>>",
-    ("grad", "z"): "k_pass<4, 64, 256, false, PassGrad<1, false>>",
-    ("hyper_div", "x"): "k_pass<4, 64, 1024, false, PassHyperDiv>",
-    ("hyper_div", "z"): "k_pass<4, 64, 1024, false, PassHyperDiv>",
-    ("hyper_div", "y"): "k_pass<4, 64, 256, false, PassHyperDiv>",
-    "cross": "k_cross",
-    "rk_filter_x": "k_pass<4, 64, 1024, false, PassFilterQ<0>>",
-    "mixh_y": "k_pass<4, 64, 256, false, PassMix<1, true>>",
-    "mixh_z": "k_pass<4, 64, 512, false, PassMix<2, true>>",
+# kernel template of each pass kind (csrc/st_pass_*.hip)
+PASS_STRUCT = {
+    "mix_y": "PassMix<1, false>", "mix_z": "PassMix<2, false>", "mixh_x": "PassMix<0, true>",
+    "mixh_y": "PassMix<1, true>", "mixh_z": "PassMix<2, true>", "lap2": "PassLap2<false>",
+    "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta<false>", "gauss_coef": "PassGaussEta<true>",
+    "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>", "filter_y": "PassFilterQ<1>",
+    "filter_z_fix": "PassFilterQ<2>",
 }
+
+
+def kernel_symbol(kind, d, narrow, nt, n):
+    """The kernel symbol (as rocprofv3 prints it) that runs pass kind.d at
+    lines of n points (n a power of two, 16..256) with the given layout."""
+    if kind == "cross":
+        return "k_cross"
+    M = 8 if n >= 64 else max(2, n // 8)
+    P = n // M
+    if narrow and 2 * P <= 64:
+        M, P = M // 2, 2 * P
+    if kind == "grad":
+        st = "PassGrad<1, true>" if nt == 1024 else ("PassGrad<3, false>" if d == "y" else "PassGrad<1, false>")
+    else:
+        st = PASS_STRUCT.get(kind)
+        if st is None:
+            return kind
+    return "k_pass<%d, %d, %d, false, %s>" % (M, P, nt, st)
 
 
 def traffic_of(kernel):
@@ -184,6 +197,7 @@ def main():
     el = time.perf_counter() - t0
     # per-pass HIP-event timing in a separate run of the same steps, so
     # that the events do not perturb the timed region
+    cfgs = s.pass_configs()
     prof = {}
     if args.profile:
         s.profile(True)
@@ -218,7 +232,7 @@ def main():
             if not alg:
                 continue
             kind, d = name.rsplit(".", 1)
-            key = KERNEL_SYMBOL.get((kind, d), KERNEL_SYMBOL.get(kind, kind))
+            key = kernel_symbol(kind, d, *cfgs.get(name, (0, 256)), n)
             g = groups.setdefault(key, {"ms": 0.0, "count": 0, "bytes": 0.0, "names": []})
             g["ms"] += p["ms"]
             g["count"] += p["count"]

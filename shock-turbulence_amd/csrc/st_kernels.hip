@@ -73,35 +73,31 @@ __global__ void __launch_bounds__(256) k_cross(PassArgs a0, int hyp)
 
 void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
 {
-    // Measured per pass at 256^3.  Wider tiles (more lines, so longer
-    // unit-stride rows in the strided x and z sweeps) where the pass's
-    // registers allow the waves; M/2 points per lane ("narrow") where the
-    // pass holds many fields in registers.
-    *narrow = 1;
+    // Chosen by st_autotune at 256^3 (per pass and direction): wider tiles
+    // (more lines, so longer unit-stride rows in the strided x and z sweeps)
+    // where the pass's registers allow the waves; M/2 points per lane
+    // ("narrow") where the pass holds many fields in registers.
+    struct C { int n, t; };
+    C c = {0, 256};
     switch (kind) {
-    case K_GRAD:
-        *nt = dir == 0 ? 1024 : 256;
-        break;
-    case K_MIX_Y: case K_MIXH_Y: case K_F_Y:
-        *nt = 256;
-        break;
-    case K_MIX_Z: case K_MIXH_Z: case K_GAUSS_ETA: case K_F_Z:
-        *nt = 512;
-        break;
-    case K_HDIV:
-        *nt = dir == 1 ? 256 : 1024;
-        break;
-    case K_MIXH_X: case K_GAUSS_ETA_LAST: case K_RKF_X:
-        *nt = 1024;
-        break;
-    case K_LAP2: case K_LAP2_LAST:
-        *narrow = 0;
-        *nt = 1024;
-        break;
-    default:
-        *narrow = 0;
-        *nt = 256;
+    case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_MIX_Y: c = {1, 256}; break;
+    case K_MIX_Z: c = {1, 512}; break;
+    case K_MIXH_X: c = {0, 512}; break;
+    case K_MIXH_Y: c = {1, 512}; break;
+    case K_MIXH_Z: c = {1, 512}; break;
+    case K_LAP2: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_LAP2_LAST: c = {0, 1024}; break;
+    case K_GAUSS_ETA: c = dir == 0 ? C{0, 256} : C{0, 512}; break;
+    case K_GAUSS_ETA_LAST: c = {1, 1024}; break;
+    case K_HDIV: c = dir == 2 ? C{1, 1024} : C{0, 512}; break;
+    case K_RKF_X: c = {0, 512}; break;
+    case K_F_Y: c = {0, 256}; break;
+    case K_F_Z: c = {0, 256}; break;
+    default: break;
     }
+    *narrow = c.n;
+    *nt = c.t;
 }
 
 hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)

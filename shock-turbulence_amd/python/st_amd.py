@@ -299,6 +299,10 @@ class Solver:
     def autotune(self, reps=2):
         """Per-pass lane layout and tile width by timing; returns {pass: (narrow, nt)}."""
         self._chk(lib().st_autotune(self.h, reps), "st_autotune")
+        return self.pass_configs()
+
+    def pass_configs(self):
+        """{"kind.direction": (narrow, threads per workgroup)} of every pass."""
         out = {}
         for k in range(72):
             name = lib().st_pass_name(k).decode()

@@ -13,7 +13,8 @@ max|gpu - cpu| / max|cpu| per field:
                                under 1-ulp perturbations of Q where that is
                                larger
   artificial coefficients      1e-11, or three times that spread
-  20 steps                     1e-10, or three times the oracle's spread
+  20 steps                     1e-10, or ten times the oracle's spread
+                               (at most 1e-8)
 The one-sided rows cancel strongly (the D2 row 1 sums |coefficients| 56
 times its result, and |d4 e| passes two of them), so the reference's own
 result moves by more than 1e-12 when Q moves by an ulp once the lines are
@@ -165,4 +166,7 @@ def test_np_steps_20(solvers):
     _log("np 20 steps relerr", e, "spread", spread, "dt", dts_g[0], dts_c[0])
     assert st_g == st_c == 20
     assert abs(t_g - t_c) <= 1e-10 * t_c
-    assert e.max() < max(1e-10, 3 * spread)
+    # the GPU differs from the oracle by a few ulps at every operation, not
+    # only at the start, so its distance after 20 steps is a small multiple
+    # of the single-perturbation spread
+    assert e.max() < min(1e-8, max(1e-10, 10 * spread))

@@ -70,7 +70,7 @@ def test_autotune_keeps_results():
             s.upload(Q)
             s.march(3)
             from fields import relerr
-            assert relerr(b.store(0), s.download()).max() < 1e-12
+            assert relerr(b.store(0), s.download()).max() < 1e-11    # 3 steps (33 stages)
         finally:
             s.close()
     finally:
