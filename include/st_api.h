@@ -155,6 +155,38 @@ int st_query(st_ctx *ctx);
 int st_autotune(st_ctx *ctx, int reps);
 int st_pass_config(st_ctx *ctx, int kind_dir, int *narrow, int *nt);
 
+/* ---- on-disk formats (host only, no context) ----
+ * Write_Restart / Read_Restart  src/out.f:493-692: a 28-byte header (the
+ * extents Ny, Nz, Nx, 5 as int32, the time as float64, the step as int32)
+ * then Q(Ny,Nz,Nx,5) as float64 in Fortran order, native byte order (one
+ * rank: the whole grid).  st_restart_read fails (-4) on a file of other
+ * extents.  The reference names the files <restart_path>/<prefix>.restart.<n>
+ * (:528-532), reading n = 0 at start-up (src/main.f:69-71) and writing
+ * n = 1, 2, ... (src/out.f:722-725). */
+int st_restart_write(const char *path, int nx, int ny, int nz, const double *Q, double time, int step);
+int st_restart_read(const char *path, int nx, int ny, int nz, double *Q, double *time, int *step);
+
+/* The input deck of Define_Parameters (src/define_periodic_sf.f:20-80), in
+ * its record order: a comment line per section, list-directed values (the
+ * first token of each line, d exponents allowed), paths as A20, and a
+ * blank record between sections.  Grid extents are not in the deck (they
+ * are compile-time in the reference, src/header:5-10). */
+typedef struct {
+    char vis_path[24], restart_path[24], stats_path[24], probe_path[24], source_path[24], prefix[24];
+    int grid_x, grid_y;
+    double x_start, y_start, z_start, period_x, period_y, period_z;
+    double Re, Pr, Sc, gamma, vis_exp, bulk_ratio, T_ratio;
+    int hyper_param, part_param;
+    int march_mode;
+    double Tstep, cfl_cal;
+    int Nsteps;
+    double run_time;
+    int comp_stats;
+    int out_grid, out_init, out_incr_x, out_incr_y, out_incr_z, out_restart, out_vis, out_probe, out_stats,
+        out_source;
+} st_deck;
+int st_deck_read(const char *path, st_deck *deck);
+
 /* Seeded isotropic-turbulence initial condition (new; the reference restarts
  * from a file, src/main.f:69-71).  Q(My,Mz,Mx,5) on the host. */
 int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,

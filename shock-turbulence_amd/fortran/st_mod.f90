@@ -15,6 +15,22 @@ module st_capi
      integer(c_int) :: nonperiodic_x = 0
   end type st_grid
 
+  ! The input deck of Define_Parameters (src/define_periodic_sf.f:20-80).
+  type, bind(C) :: st_deck
+     character(kind=c_char) :: vis_path(24), restart_path(24), stats_path(24), probe_path(24), &
+          source_path(24), prefix(24)
+     integer(c_int) :: grid_x, grid_y
+     real(c_double) :: x_start, y_start, z_start, period_x, period_y, period_z
+     real(c_double) :: Re, Pr, Sc, gamma, vis_exp, bulk_ratio, T_ratio
+     integer(c_int) :: hyper_param, part_param, march_mode
+     real(c_double) :: Tstep, cfl_cal
+     integer(c_int) :: Nsteps
+     real(c_double) :: run_time
+     integer(c_int) :: comp_stats
+     integer(c_int) :: out_grid, out_init, out_incr_x, out_incr_y, out_incr_z, out_restart, out_vis, &
+          out_probe, out_stats, out_source
+  end type st_deck
+
   type, bind(C) :: st_params
      real(c_double) :: Re, Pr, gamma, vis_exp, bulk_ratio
      integer(c_int) :: hyper_param
@@ -147,9 +163,52 @@ module st_capi
        integer(c_long_long), value :: seed
        real(c_double), intent(inout) :: Q(*)
      end function st_ic_isotropic
+
+     integer(c_int) function st_restart_write(path, nx, ny, nz, Q, time, step) bind(C, name='st_restart_write')
+       import :: c_int, c_double, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: nx, ny, nz
+       real(c_double), intent(in) :: Q(*)
+       real(c_double), value :: time
+       integer(c_int), value :: step
+     end function st_restart_write
+
+     integer(c_int) function st_restart_read(path, nx, ny, nz, Q, time, step) bind(C, name='st_restart_read')
+       import :: c_int, c_double, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: nx, ny, nz
+       real(c_double), intent(inout) :: Q(*)
+       real(c_double), intent(out) :: time
+       integer(c_int), intent(out) :: step
+     end function st_restart_read
+
+     integer(c_int) function st_deck_read(path, deck) bind(C, name='st_deck_read')
+       import :: c_int, c_char, st_deck
+       character(kind=c_char), intent(in) :: path(*)
+       type(st_deck), intent(out) :: deck
+     end function st_deck_read
   end interface
 
 contains
+
+  ! A Fortran string as a NUL-terminated C string.
+  function c_str(f) result(c)
+    character(len=*), intent(in) :: f
+    character(kind=c_char, len=len_trim(f) + 1) :: c
+    c = trim(f) // c_null_char
+  end function c_str
+
+  ! A NUL-terminated C character array as a Fortran string.
+  function f_str(c) result(f)
+    character(kind=c_char), intent(in) :: c(:)
+    character(len=size(c)) :: f
+    integer :: i
+    f = ' '
+    do i = 1, size(c)
+       if (c(i) == c_null_char) exit
+       f(i:i) = c(i)
+    end do
+  end function f_str
 
   ! Message of the last failure on ctx, as a Fortran string.
   function st_error_text(ctx) result(msg)

@@ -9,6 +9,16 @@
 ! Re <= 0 the Reynolds number is set from re_lambda at t = 0:
 ! Re = (1/M_t^2)^vis_exp / (u_rms (2/k0) / Re_lambda).
 !
+! With deck set, the run settings come from an input deck in the
+! reference's format (src/define_periodic_sf.f:20-80): periods, starts, Re,
+! Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal and Nsteps; and, as
+! in src/main.f:69-71, the state starts from <restart_path>/<prefix>.restart.0
+! when that file exists.  restart_in names a restart file to start from
+! explicitly.  With restart_every > 0, restart files
+! <restart_dir>/<prefix>.restart.<n>, n = 1, 2, ... are written every that
+! many steps (src/out.f:722-725; the reference writes every 50), in the
+! format of Write_Restart (src/out.f:493-591).
+!
 ! Output: the log columns of src/main.f:109-116 that this build computes
 ! (MaxDil and Enstrophy of Compute_Grad are not), in the reference format
 ! (1x,i5,ES13.5) appended to the file 'output', and the same values at full
@@ -23,17 +33,21 @@ program st_run
   real(8) :: k0 = 4.0d0, urms = 1.0d0, mt = 0.3d0, re_lambda = 40.0d0, Re = 0.0d0, Pr = 0.7d0
   real(8) :: gamma = 1.4d0, vis_exp = 0.76d0, bulk_ratio = 0.0d0, cfl_cal = 0.5d0
   real(8) :: period = 6.283185307179586d0
-  character(len=256) :: qout = ' '
+  character(len=256) :: qout = ' ', deck = ' ', restart_in = ' ', restart_dir = '.', prefix = 'st'
+  integer :: restart_every = 0
   namelist /run/ nx, ny, nz, nsteps, hyper_param, d2_mode, rk_scheme, seed, k0, urms, mt, &
-       re_lambda, Re, Pr, gamma, vis_exp, bulk_ratio, cfl_cal, period, qout
+       re_lambda, Re, Pr, gamma, vis_exp, bulk_ratio, cfl_cal, period, qout, deck, restart_in, &
+       restart_dir, prefix, restart_every
 
   type(st_grid) :: grid
   type(st_params) :: params
   real(8), allocatable :: Q(:, :, :, :)
-  real(8) :: time, cfl_stab, cfl_resv, per(3)
+  real(8) :: time, cfl_stab, cfl_resv, per(3), start(3)
   real(8) :: MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy
-  integer :: step, step_count, u, ios
-  character(len=256) :: arg
+  integer :: step, step_count, u, ios, cue
+  character(len=256) :: arg, fname
+  type(st_deck) :: dk
+  logical :: have
 
   if (command_argument_count() >= 1) then
      call get_command_argument(1, arg)
@@ -45,19 +59,46 @@ program st_run
      read(u, nml=run)
      close(u)
   end if
-  if (Re <= 0.0d0) Re = (1.0d0 / (mt * mt)) ** vis_exp / (urms * (2.0d0 / k0) / re_lambda)
+  per = period
+  start = 0.0d0
+  if (len_trim(deck) > 0) then
+     call st_check(st_deck_read(c_str(deck), dk), 'st_deck_read ' // trim(deck))
+     per = [dk%period_x, dk%period_y, dk%period_z]
+     start = [dk%x_start, dk%y_start, dk%z_start]
+     Re = dk%Re
+     Pr = dk%Pr
+     gamma = dk%gamma
+     vis_exp = dk%vis_exp
+     bulk_ratio = dk%bulk_ratio
+     hyper_param = dk%hyper_param
+     cfl_cal = dk%cfl_cal
+     nsteps = dk%Nsteps
+     prefix = f_str(dk%prefix)
+     restart_dir = f_str(dk%restart_path)
+     if (len_trim(restart_in) == 0) then
+        fname = trim(restart_dir) // '/' // trim(prefix) // '.restart.0'
+        inquire(file=trim(fname), exist=have)
+        if (have) restart_in = fname
+     end if
+  else if (Re <= 0.0d0) then
+     Re = (1.0d0 / (mt * mt)) ** vis_exp / (urms * (2.0d0 / k0) / re_lambda)
+  end if
 
-  grid = st_grid(nx, ny, nz, period, period, period, 0.0d0, 0.0d0, 0.0d0)
+  grid = st_grid(nx, ny, nz, per(1), per(2), per(3), start(1), start(2), start(3))
   params = st_params(Re, Pr, gamma, vis_exp, bulk_ratio, hyper_param, cfl_cal, d2_mode, rk_scheme)
   call Setup_Solver(grid, params)
 
   allocate(Q(ny, nz, nx, 5))
-  per = period
-  call st_check(st_ic_isotropic(nx, ny, nz, per, k0, urms, mt, gamma, seed, Q), 'st_ic_isotropic')
-  call Upload_State(Q)
-
   time = 0.0d0
   step = 0
+  if (len_trim(restart_in) > 0) then
+     call st_check(st_restart_read(c_str(restart_in), nx, ny, nz, Q, time, step), &
+          'st_restart_read ' // trim(restart_in))
+  else
+     call st_check(st_ic_isotropic(nx, ny, nz, per, k0, urms, mt, gamma, seed, Q), 'st_ic_isotropic')
+  end if
+  call Upload_State(Q)
+  cue = 1
   call Compute_CFL(Q, cfl_stab, cfl_resv, step)
   call Compute_Tstep(Q, cfl_resv, step)
   call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
@@ -70,6 +111,15 @@ program st_run
      call LDDRK4(time, step)
      call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
      call log_line()
+     if (restart_every > 0) then
+        if (mod(step, restart_every) == 0) then
+           call Download_State(Q)
+           write(fname, '(a,a,a,a,i0)') trim(restart_dir), '/', trim(prefix), '.restart.', cue
+           call st_check(st_restart_write(c_str(fname), nx, ny, nz, Q, time, step), &
+                'st_restart_write ' // trim(fname))
+           cue = cue + 1
+        end if
+     end if
   end do
 
   if (len_trim(qout) > 0) then

@@ -33,6 +33,20 @@ class Grid(ctypes.Structure):
                 ("nonperiodic_x", ctypes.c_int)]
 
 
+class Deck(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_char * 24) for n in ("vis_path", "restart_path", "stats_path", "probe_path",
+                                                   "source_path", "prefix")]
+                + [("grid_x", ctypes.c_int), ("grid_y", ctypes.c_int)]
+                + [(n, ctypes.c_double) for n in ("x_start", "y_start", "z_start", "period_x", "period_y",
+                                                  "period_z", "Re", "Pr", "Sc", "gamma", "vis_exp", "bulk_ratio",
+                                                  "T_ratio")]
+                + [("hyper_param", ctypes.c_int), ("part_param", ctypes.c_int), ("march_mode", ctypes.c_int),
+                   ("Tstep", ctypes.c_double), ("cfl_cal", ctypes.c_double), ("Nsteps", ctypes.c_int),
+                   ("run_time", ctypes.c_double), ("comp_stats", ctypes.c_int)]
+                + [(n, ctypes.c_int) for n in ("out_grid", "out_init", "out_incr_x", "out_incr_y", "out_incr_z",
+                                               "out_restart", "out_vis", "out_probe", "out_stats", "out_source")])
+
+
 class Params(ctypes.Structure):
     _fields_ = [("Re", ctypes.c_double), ("Pr", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("vis_exp", ctypes.c_double), ("bulk_ratio", ctypes.c_double),
@@ -47,6 +61,7 @@ EXPORTS = [
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_query",
+    "st_restart_write", "st_restart_read", "st_deck_read",
 ]
 
 _lib = None
@@ -104,6 +119,10 @@ def lib():
     L.st_batch_status.argtypes = [vp, ip, _dp, _dp, ip]
     L.st_autotune.argtypes = [vp, ctypes.c_int]
     L.st_query.argtypes = [vp]
+    L.st_restart_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                   ctypes.c_int]
+    L.st_restart_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ip]
+    L.st_deck_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(Deck)]
     L.st_pass_config.argtypes = [vp, ctypes.c_int, ip, ip]
     _lib = L
     return L
@@ -119,6 +138,39 @@ def _p(a):
 
 def _f(a):
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def restart_write(path, Q, time, step):
+    """Write_Restart format (src/out.f:493-591) for Q(Ny,Nz,Nx,5)."""
+    Q = _f(Q)
+    ny, nz, nx, _ = Q.shape
+    r = lib().st_restart_write(str(path).encode(), nx, ny, nz, _p(Q), time, step)
+    if r:
+        raise StError("st_restart_write failed (%d)" % r)
+
+
+def restart_read(path, nx, ny, nz):
+    """Read_Restart (src/out.f:598-692): returns (Q, time, step)."""
+    Q = np.zeros((ny, nz, nx, 5), dtype=np.float64, order="F")
+    t = np.zeros(1)
+    st = ctypes.c_int()
+    r = lib().st_restart_read(str(path).encode(), nx, ny, nz, _p(Q), _p(t), ctypes.byref(st))
+    if r:
+        raise StError("st_restart_read failed (%d)" % r)
+    return Q, float(t[0]), st.value
+
+
+def deck_read(path):
+    """Define_Parameters' input deck (src/define_periodic_sf.f:20-80) as a dict."""
+    d = Deck()
+    r = lib().st_deck_read(str(path).encode(), ctypes.byref(d))
+    if r:
+        raise StError("st_deck_read failed (%d)" % r)
+    out = {}
+    for name, _ in Deck._fields_:
+        v = getattr(d, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
 
 
 def ic_isotropic(nx, ny, nz, k0=4.0, urms=1.0, mt=0.3, gamma=1.4, seed=12345, period=(2 * np.pi,) * 3):

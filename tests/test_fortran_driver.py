@@ -51,3 +51,44 @@ def test_driver_march_matches_oracle(tmp_path):
     assert abs(last[0] - t_c) <= 1e-12 * t_c
     assert abs(last[1] - dts[-1]) <= 1e-11 * dts[-1]
     np.testing.assert_allclose(last[3:], o.diagnostics(Qc)[[3, 0, 1, 2, 4, 5, 6]], rtol=1e-10, atol=1e-13)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_driver_deck_restart_resume(tmp_path):
+    """A reference-format deck and restart files: a run interrupted after its
+    restart at step 2 and resumed from that file (as src/main.f:69-71 reads
+    <prefix>.restart.0) ends with the same bits as the uninterrupted run."""
+    st_amd = pytest.importorskip("st_amd")
+    from test_io_formats import DECK
+    n = 32
+    deck = DECK.replace("458.9d0              Re", "200d0                Re") \
+               .replace("200                  Nsteps", "4                    Nsteps") \
+               .replace("3.1415926535897931d0 period_z", "6.283185307179586d0  period_z")
+    for d in ("full", "part"):
+        (tmp_path / d / "restart").mkdir(parents=True)
+        (tmp_path / d / "cit.input.1").write_text(deck)
+    # both start from the same restart.0 (a seeded IC written in the reference format)
+    Q0 = st_amd.ic_isotropic(n, n, n, k0=4.0, urms=1.0, mt=0.3, seed=99)
+    for d in ("full", "part"):
+        st_amd.restart_write(tmp_path / d / "restart" / "iso.restart.0", Q0, 0.0, 0)
+    (tmp_path / "full" / "run.nml").write_text("&run nx=%d, ny=%d, nz=%d, deck='cit.input.1', restart_every=2, "
+                                               "qout='q.bin' /\n" % (n, n, n))
+    r = subprocess.run([EXE, "run.nml"], cwd=tmp_path / "full", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    Q2, t2, s2 = st_amd.restart_read(tmp_path / "full" / "restart" / "iso.restart.1", n, n, n)
+    assert s2 == 2 and t2 > 0
+    # the resumed run: 2 steps from the step-2 restart
+    st_amd.restart_write(tmp_path / "part" / "restart" / "iso.restart.0", Q2, t2, s2)
+    (tmp_path / "part" / "run.nml").write_text("&run nx=%d, ny=%d, nz=%d, deck='cit.input.1', nsteps=2, "
+                                               "qout='q.bin' /\n" % (n, n, n))
+    # nsteps in the namelist is overridden by the deck's Nsteps = 4; use a deck with 2
+    (tmp_path / "part" / "cit.input.1").write_text(deck.replace("4                    Nsteps",
+                                                                "2                    Nsteps"))
+    r = subprocess.run([EXE, "run.nml"], cwd=tmp_path / "part", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    a = np.fromfile(tmp_path / "full" / "q.bin", dtype=np.float64)
+    b = np.fromfile(tmp_path / "part" / "q.bin", dtype=np.float64)
+    assert np.array_equal(a, b)
+    Q4, t4, s4 = st_amd.restart_read(tmp_path / "full" / "restart" / "iso.restart.2", n, n, n)
+    assert s4 == 4 and np.array_equal(Q4.ravel(order="F"), a)
