@@ -122,6 +122,43 @@ def ensemble_leg(nsteps, autotune):
             "speedup": round(plain / el, 3), "autotune_reps": autotune}
 
 
+def config2_leg(nsteps, warmup):
+    """512x192x192 with a non-periodic x: a Mach-1.5 shock at x = L/4 with
+    upstream turbulence and Rankine-Hugoniot sponge layers (20% of x at
+    each end), advanced with the non-periodic link set."""
+    import math
+    import numpy as np
+    import st_amd
+    nx, ny, nz = 512, 192, 192
+    d = 2 * math.pi / ny
+    per = (d * (nx - 1), 2 * math.pi, 2 * math.pi)
+    t0 = time.perf_counter()
+    Q, s1, s2 = st_amd.ic_shock(nx, ny, nz, per, mach=1.5, frac=0.25, k0=4.0, urms=0.1, seed=2)
+    t_ic = time.perf_counter() - t0
+    s = st_amd.Solver(nx, ny, nz, period=per, nonperiodic_x=1, Re=500.0, Pr=0.7, hyper_param=1, cfl_cal=0.5)
+    try:
+        s.sponge(5.0, 2.0, 0.4, s1, s2)
+        s.upload(Q)
+        t, st, _ = s.march(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        t, st, dts = s.march(nsteps, t, st)
+        s.sync()
+        el = time.perf_counter() - t0
+        finite = bool(np.isfinite(s.download()).all())
+    finally:
+        s.close()
+    pts = float(nx * ny * nz)
+    return {"workload": "512x192x192, x non-periodic (closures, mirrored Gaussian, constant hyper coefficients), "
+                        "Mach 1.5 shock at x=L/4, upstream turbulence k0=4 u_rms=0.1, sponge amp 5 power 2 over 20% "
+                        "of x at each end toward the Rankine-Hugoniot states, Re=500, Pr=0.7, CFL 0.5",
+            "steps": nsteps, "warmup": warmup, "ic_seconds": round(t_ic, 2),
+            "mpts_stage_per_s": round(pts * nsteps * STAGES_PER_STEP / el / 1e6, 1),
+            "ms_per_step": round(el / nsteps * 1e3, 2),
+            "roofline_stage_frac": round(B_ALG * pts * nsteps * STAGES_PER_STEP / el / 1e9 / HBM_PEAK_GBS, 4),
+            "finite": finite, "dt_last": float(dts[-1])}
+
+
 def serving_leg(nruns, rate, seed):
     """A finite Poisson trace of runs (sizes 48^3..128^3, 50..400 steps)
     served with continuous batching, against the plain path one at a time
@@ -156,6 +193,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=6, help="batched-ensemble leg (0: skip)")
     ap.add_argument("--autotune", type=int, default=0, help="autotune the ensemble batches (stages per candidate)")
     ap.add_argument("--serve-runs", type=int, default=16, help="serving leg: runs in the trace (0: skip)")
+    ap.add_argument("--config2-steps", type=int, default=3, help="non-periodic 512x192x192 leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 2 GiB fill and a 2 GiB copy first (reference kernels for rocprofv3 byte counters)")
@@ -256,6 +294,7 @@ def main():
     del Q, Qend
     ensemble = ensemble_leg(args.ensemble_steps, args.autotune) if args.ensemble_steps > 0 else None
     serving = serving_leg(args.serve_runs, args.serve_rate, 2024) if args.serve_runs > 0 else None
+    config2 = config2_leg(args.config2_steps, 1) if args.config2_steps > 0 else None
 
     out = {
         "metric": "Mpts*RK-stage/s FP64 (256^3 periodic RHS+LDDRK4 stage throughput)",
@@ -280,6 +319,7 @@ def main():
         "kernels": kernels,
         "ensemble": ensemble,
         "serving": serving,
+        "nonperiodic_512x192x192": config2,
         "check": {"finite": finite, "sim_time": time_, "step": step, "dt_last": float(dts[-1]) if len(dts) else None,
                   "diagnostics": [float(x) for x in diag]},
     }

@@ -187,6 +187,22 @@ typedef struct {
 } st_deck;
 int st_deck_read(const char *path, st_deck *deck);
 
+/* Sponge_Forcing  src/sponge1d.f:7-95 in every stage after Hyper_RHS (the
+ * hook order of src/lddrk4_nonperiodic_mf.f:462-466): dQ -= sigma(x)
+ * (Q - Qref) with sigma = amp (d/len)^power over the first and last
+ * ceil(frac nx/2) x points (the reference: amp 5, power 2, frac 0.4).  The
+ * reference holds one hard-coded Qref (:51-57); here the left layer relaxes
+ * to ref_left and the right to ref_right (conserved variables).  amp = 0
+ * turns it off. */
+int st_sponge(st_ctx *ctx, double amp, double power, double frac, const double ref_left[5],
+              const double ref_right[5]);
+/* Planar Mach-`mach` shock at x = frac*period_x (tanh profile of the given
+ * width) on the non-periodic x grid, Rankine-Hugoniot states on either
+ * side, upstream isotropic turbulence (k0, urms; urms = 0: none).  states
+ * (may be NULL) receives the upstream and downstream conserved states (10). */
+int st_ic_shock(int nx, int ny, int nz, const double period[3], double mach, double frac, double width,
+                double k0, double urms, double gamma, unsigned long long seed, double *Q, double *states);
+
 /* Seeded isotropic-turbulence initial condition (new; the reference restarts
  * from a file, src/main.f:69-71).  Q(My,Mz,Mx,5) on the host. */
 int st_ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,

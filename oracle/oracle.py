@@ -53,6 +53,7 @@ def _load():
     lib.or_band_solve.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_long]
     lib.or_np_lhs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     lib.or_np_band_solve.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_long]
+    lib.or_set_sponge.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
     lib.or_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.or_compute_hyper.argtypes = [ctypes.c_void_p, _dp]
     lib.or_hyper_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -140,6 +141,12 @@ class Oracle:
         r = np.zeros((self.params.nx, 5))
         lib().or_np_lhs(self.h, op, r.ctypes.data_as(_dp))
         return r
+
+    def set_sponge(self, amp, power, frac, ref_left, ref_right):
+        """Sponge_Forcing after Hyper_RHS in every stage (amp 0: off)."""
+        L = np.ascontiguousarray(ref_left, dtype=np.float64)
+        R = np.ascontiguousarray(ref_right, dtype=np.float64)
+        lib().or_set_sponge(self.h, amp, power, frac, L.ctypes.data_as(_dp), R.ctypes.data_as(_dp))
 
     def metrics(self):
         out = np.zeros(9)

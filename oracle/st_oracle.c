@@ -255,6 +255,7 @@ struct or_ctx {
     double c2_d2, d2_d2;        /* D2 +-3 weight and centre weight */
     band_t *b1[3], *b2[3], *bf[3];
     double *np_rows[3];         /* non-periodic x: LHS rows of D1, D2, filter (5*Mx each) */
+    double sp_amp, sp_pow, sp_frac, sp_ref[10];   /* sponge (amp 0: off) */
     double *grad;               /* /GRAD/: dux_dx duy_dx duz_dx dux_dy duy_dy duz_dy
                                    dux_dz duy_dz duz_dz dTdx dTdy dTdz */
     double *hyp;                /* /HYPER/: beta_hyper, visc_hyper, k_hyper */
@@ -1378,11 +1379,53 @@ double or_compute_tstep(or_ctx *c, const double *Q, double cfl_resv, int step)
 }
 
 /* one stage body :344-356: RHS, Hyper_RHS, 2N-storage update, filter, fix */
+void or_set_sponge(or_ctx *c, double amp, double power, double frac, const double *ref_left, const double *ref_right)
+{
+    c->sp_amp = amp;
+    c->sp_pow = power;
+    c->sp_frac = frac;
+    for (int l = 0; l < 5; l++) {
+        c->sp_ref[l] = ref_left ? ref_left[l] : 0.0;
+        c->sp_ref[5 + l] = ref_right ? ref_right[l] : 0.0;
+    }
+}
+
+/* Sponge_Forcing src/sponge1d.f:29-95 (one rank: gI = I); the left layer
+   relaxes to the first reference state, the right layer to the second */
+static void sponge(or_ctx *c, const double *Q, double *Qrhs)
+{
+    const int Nx = c->n[0];
+    const long plane = (long)c->n[1] * c->n[2];
+    const long N = c->npts;
+    const int I_left = (int)ceil((c->sp_frac * Nx) / 2.0);          /* :35 */
+    const int I_right = Nx - I_left;                                 /* :36 */
+    const int I_len = I_left;                                        /* :38 */
+#pragma omp parallel for schedule(static)
+    for (int I = 1; I <= Nx; I++) {
+        double sigma = 0.0;
+        const double *ref = c->sp_ref;
+        if (I <= I_left) {                                           /* :65-68 */
+            const float r = (float)(I_left - I + 1) / (float)I_len;
+            sigma = c->sp_amp * pow((double)r, c->sp_pow);
+        } else if (I >= I_right) {                                   /* :69-72 */
+            const float r = (float)(I - I_right) / (float)I_len;
+            sigma = c->sp_amp * pow((double)r, c->sp_pow);
+            ref = c->sp_ref + 5;
+        }
+        for (long p = 0; p < plane; p++) {
+            const long l = (long)(I - 1) * plane + p;
+            for (int L = 0; L < 5; L++)                                 /* :85-95 */
+                Qrhs[L * N + l] = Qrhs[L * N + l] - (sigma) * (Q[L * N + l] - ref[L]);
+        }
+    }
+}
+
 void or_rk_stage(or_ctx *c, double *Q, double *QW, double *dQ, double alpha, double beta, double dt)
 {
     const size_t F5 = (size_t)5 * c->npts;
     or_rhs(c, Q, dQ);
     or_hyper_rhs(c, Q, dQ);
+    if (c->sp_amp != 0.0) sponge(c, Q, dQ);       /* hook order of src/lddrk4_nonperiodic_mf.f:462-466 */
 #pragma omp parallel for schedule(static)
     for (size_t l = 0; l < F5; l++) {
         QW[l] = alpha * QW[l] + dt * dQ[l];                  /* :349 */

@@ -78,6 +78,9 @@ void or_fix_pressure(or_ctx *c, double *Q);
 void or_compute_cfl(or_ctx *c, const double *Q, double *stab, double *resv);
 double or_compute_tstep(or_ctx *c, const double *Q, double cfl_resv, int step);
 void or_rk_stage(or_ctx *c, double *Q, double *QW, double *dQ, double alpha, double beta, double dt);
+/* Sponge_Forcing (src/sponge1d.f) applied after Hyper_RHS in every stage;
+   amp = 0 turns it off */
+void or_set_sponge(or_ctx *c, double amp, double power, double frac, const double *ref_left, const double *ref_right);
 void or_lddrk4(or_ctx *c, double *Q, double dt, double *time, int *step);
 /* one march-mode-1 iteration of src/main.f:129-136: CFL, dt, LDDRK4 */
 double or_march(or_ctx *c, double *Q, double *time, int *step);

@@ -58,6 +58,8 @@ struct st_ctx {
     std::vector<int> act_h;
     int nact = 0;                   // active slots, listed in s_list (launch grids cover these only)
     std::vector<hipGraphExec_t> graphs;   // one batch step, captured per active-slot count
+    double *spg = nullptr;          // sponge: sigma per x index, then the left and right reference states
+    int spg_left = 0;               // x indices [0, spg_left) take the left reference state
     int use_graph = 1;
     hipStream_t s = nullptr;
     char err[512];
@@ -429,6 +431,7 @@ extern "C" void st_ctx_destroy(st_ctx *c)
     for (auto g : c->graphs)
         if (g) (void)hipGraphExecDestroy(g);
     if (c->s_re) (void)hipFree(c->s_re);
+    if (c->spg) (void)hipFree(c->spg);
     if (c->buf) (void)hipFree(c->buf);
     if (c->td) (void)hipFree(c->td);
     if (c->red) (void)hipFree(c->red);
@@ -579,6 +582,10 @@ int seq_stage(st_ctx *c, const st::PassArgs &base, double ra, double rb, double 
 {
     TRY(seq_rhs(c, base));
     TRY(seq_hyper(c, base));
+    if (c->spg) {                                            // the hook order of src/lddrk4_nonperiodic_mf.f:462-466
+        const hipError_t e = st::run_sponge(base, c->spg, c->Mx, c->spg_left, c->s);
+        if (e != hipSuccess) return fail(c, -4, "sponge: %s", hipGetErrorString(e));
+    }
     st::PassArgs a = base;
     a.rk_a = ra;
     a.rk_b = rb;
@@ -1090,6 +1097,41 @@ extern "C" int st_autotune(st_ctx *c, int reps)
     if (rc) return rc;
     if (e != hipSuccess) return fail(c, -3, "autotune restore: %s", hipGetErrorString(e));
     return 0;
+}
+
+extern "C" int st_sponge(st_ctx *c, double amp, double power, double frac, const double ref_left[5],
+                         const double ref_right[5])
+{
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (amp == 0.0) {
+        if (c->spg) (void)hipFree(c->spg);
+        c->spg = nullptr;
+    } else {
+        if (!ref_left || !ref_right || !(frac > 0.0) || frac >= 1.0) return fail(c, -1, "st_sponge: bad arguments");
+        std::vector<double> h(c->Mx + 10);
+        for (int i = 0; i < c->Mx; i++) h[i] = st::sponge_sigma(c->Mx, i, amp, power, frac);
+        for (int l = 0; l < 5; l++) {
+            h[c->Mx + l] = ref_left[l];
+            h[c->Mx + 5 + l] = ref_right[l];
+        }
+        c->spg_left = (int)std::ceil((frac * c->Mx) / 2.0);
+        if (!c->spg) HIPCHK(c, hipMalloc(&c->spg, sizeof(double) * (c->Mx + 10)));
+        HIPCHK(c, hipMemcpy(c->spg, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    }
+    for (auto &g : c->graphs)                                  // the captured steps change
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return 0;
+}
+
+extern "C" int st_ic_shock(int nx, int ny, int nz, const double period[3], double mach, double frac, double width,
+                           double k0, double urms, double gamma, unsigned long long seed, double *Q, double *states)
+{
+    if (!Q || !period) return -1;
+    return st::ic_shock(nx, ny, nz, period, mach, frac, width, k0, urms, gamma, seed, Q, states);
 }
 
 extern "C" int st_pass_config(st_ctx *c, int kind_dir, int *narrow, int *nt)

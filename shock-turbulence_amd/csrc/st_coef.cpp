@@ -234,6 +234,20 @@ int fill_np(OpTab *t, int op, double alpha, double beta, int n, int M, int P)
 
 }  // namespace
 
+double sponge_sigma(int nx, int i, double amp, double power, double frac)
+{
+    // src/sponge1d.f:33-36: CEILING((frac_sponge_x*Nx)/2D0) points at each end
+    const int start_left = (int)std::ceil((frac * nx) / 2.0);
+    const int start_right = nx - start_left;
+    const int len = start_left;
+    const int g = i + 1;                                   // global 1-based index (:63)
+    if (len < 1) return 0.0;
+    // :65-73: ( REAL(a)/REAL(len) )**s_exp, the quotient in single precision
+    if (g <= start_left) return amp * std::pow((double)((float)(start_left - g + 1) / (float)len), power);
+    if (g >= start_right) return amp * std::pow((double)((float)(g - start_right) / (float)len), power);
+    return 0.0;
+}
+
 int choose_layout(int n, int *M, int *P)
 {
     static const int cand[] = {8, 6, 4, 2};

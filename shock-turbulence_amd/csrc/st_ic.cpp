@@ -154,4 +154,56 @@ int ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, doub
     return 0;
 }
 
+// Planar shock with upstream turbulence for the non-periodic x grid
+// (x_i = period_x i/(nx-1)).  Upstream of x_s = frac L the gas is at rest
+// state rho1 = 1, p1 = 1/gamma (sound speed 1) moving at u1 = M; downstream
+// the Rankine-Hugoniot state of a normal shock of Mach M:
+//   rho2/rho1 = (gamma+1) M^2 / ((gamma-1) M^2 + 2),
+//   p2/p1 = 1 + 2 gamma/(gamma+1) (M^2 - 1),  u2 = u1 rho1/rho2,
+// joined by phi = (1 + tanh((x - x_s)/width))/2.  The isotropic field of
+// ic_isotropic (k0, u_rms) is added to the velocity upstream, scaled by
+// (1 - phi).  states[0..4] and [5..9] return the conserved upstream and
+// downstream states (rho u, rho v, rho w, rho, rho E).
+int ic_shock(int nx, int ny, int nz, const double period[3], double mach, double frac, double width,
+             double k0, double urms, double gamma, unsigned long long seed, double *Q, double *states)
+{
+    if (nx < 16 || mach <= 1.0 || frac <= 0.0 || frac >= 1.0 || width <= 0.0) return -1;
+    const long N = (long)nx * ny * nz;
+    std::vector<double> T(urms > 0.0 ? 5 * N : 0);
+    if (urms > 0.0) {
+        const int rc = ic_isotropic(nx, ny, nz, period, k0, urms, 1.0, gamma, seed, T.data());
+        if (rc) return rc;
+    }
+    const double M2 = mach * mach;
+    const double r1 = 1.0, p1 = 1.0 / gamma, u1 = mach;
+    const double r2 = r1 * (gamma + 1.0) * M2 / ((gamma - 1.0) * M2 + 2.0);
+    const double p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (M2 - 1.0));
+    const double u2 = u1 * r1 / r2;
+    const double L = period[0], xs = frac * L;
+    if (states) {
+        const double e1 = p1 / (gamma - 1.0) + 0.5 * r1 * u1 * u1, e2 = p2 / (gamma - 1.0) + 0.5 * r2 * u2 * u2;
+        const double s[10] = {r1 * u1, 0.0, 0.0, r1, e1, r2 * u2, 0.0, 0.0, r2, e2};
+        for (int i = 0; i < 10; i++) states[i] = s[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < N; l++) {
+        const int i = (int)(l / ((long)ny * nz));
+        const double x = L * (double)i / (double)(nx - 1);
+        const double phi = 0.5 * (1.0 + std::tanh((x - xs) / width));
+        const double rho = r1 + (r2 - r1) * phi, p = p1 + (p2 - p1) * phi;
+        double u = u1 + (u2 - u1) * phi, v = 0.0, w = 0.0;
+        if (urms > 0.0) {
+            u += (1.0 - phi) * T[l];
+            v += (1.0 - phi) * T[N + l];
+            w += (1.0 - phi) * T[2 * N + l];
+        }
+        Q[l] = rho * u;
+        Q[N + l] = rho * v;
+        Q[2 * N + l] = rho * w;
+        Q[3 * N + l] = rho;
+        Q[4 * N + l] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+    }
+    return 0;
+}
+
 }  // namespace st

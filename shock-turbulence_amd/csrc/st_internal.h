@@ -81,5 +81,13 @@ int choose_layout(int n, int *M, int *P);
 // Seeded isotropic turbulence initial condition (host).
 int ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,
                  double mt, double gamma, unsigned long long seed, double *Q);
+// Planar Mach-M shock at x = frac L with upstream turbulence (host).
+int ic_shock(int nx, int ny, int nz, const double period[3], double mach, double frac, double width,
+             double k0, double urms, double gamma, unsigned long long seed, double *Q, double *states);
+
+// Sponge strength of Sponge_Forcing (src/sponge1d.f:29-79) at x index
+// i = 0..nx-1: amp ((d/len))^power over the first and last ceil(frac nx/2)
+// points, the ratio formed in single precision as the reference's REAL().
+double sponge_sigma(int nx, int i, double amp, double power, double frac);
 
 }  // namespace st

@@ -84,4 +84,5 @@ hipError_t run_dt(const double *pc, const double *pt, int nb, int nslot, const i
                   const int *stepc, double cfl_cal, int hyper, int first_step, int *run, double *dt, double *dt_last,
                   hipStream_t s);
 hipError_t run_clock(int nslot, const int *run, const double *dt, double *time, int *stepc, int bump, hipStream_t s);
+hipError_t run_sponge(const PassArgs &a, const double *spg, int Mx, int nleft, hipStream_t s);
 }  // namespace st

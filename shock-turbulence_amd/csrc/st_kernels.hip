@@ -309,6 +309,33 @@ __global__ void k_clock(int nslot, const int *run, const double *dt, double *tim
     if (bump) stepc[slot] = stepc[slot] + 1;
 }
 
+// Sponge_Forcing (src/sponge1d.f:85-95): dQ -= sigma(x) (Q - Qref) with
+// sigma per x index in spg[0..Mx) and the reference states of the left
+// and right layers in spg[Mx..Mx+5) and spg[Mx+5..Mx+10).
+__global__ void __launch_bounds__(256) k_sponge(PassArgs a, const double *spg, int Mx, int nleft)
+{
+    const int slot = a.slots ? a.slots[blockIdx.y] : blockIdx.y;
+    if (a.act && !a.act[slot]) return;
+    const long N = a.N, off = (long)slot * a.ni, plane = a.ni / Mx;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
+        const int ix = (int)(i / plane);
+        const double sg = spg[ix];
+        if (sg == 0.0) continue;
+        const double *ref = spg + Mx + (ix < nleft ? 0 : 5);
+        const long gi = off + i;
+#pragma unroll
+        for (int l = 0; l < 5; l++) a.dQ[l * N + gi] = a.dQ[l * N + gi] - sg * (a.Q[l * N + gi] - ref[l]);
+    }
+}
+
+hipError_t run_sponge(const PassArgs &a, const double *spg, int Mx, int nleft, hipStream_t s)
+{
+    long blocks = (a.ni + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_sponge, dim3((unsigned)blocks, (unsigned)a.nslot), dim3(256), 0, s, a, spg, Mx, nleft);
+    return hipGetLastError();
+}
+
 static long red_blocks(long N)
 {
     long b = (N + 255) / 256;

@@ -61,7 +61,7 @@ EXPORTS = [
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_query",
-    "st_restart_write", "st_restart_read", "st_deck_read",
+    "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock",
 ]
 
 _lib = None
@@ -123,6 +123,10 @@ def lib():
                                    ctypes.c_int]
     L.st_restart_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ip]
     L.st_deck_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(Deck)]
+    L.st_sponge.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.st_ic_shock.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_ulonglong, _dp, _dp]
     L.st_pass_config.argtypes = [vp, ctypes.c_int, ip, ip]
     _lib = L
     return L
@@ -171,6 +175,20 @@ def deck_read(path):
         v = getattr(d, name)
         out[name] = v.decode() if isinstance(v, bytes) else v
     return out
+
+
+def ic_shock(nx, ny, nz, period, mach=1.5, frac=0.25, width=None, k0=4.0, urms=0.1, gamma=1.4, seed=12345):
+    """Planar shock with upstream turbulence on the non-periodic x grid;
+    returns (Q, upstream state, downstream state)."""
+    if width is None:
+        width = 2.0 * period[0] / (nx - 1)
+    Q = np.zeros((ny, nz, nx, 5), dtype=np.float64, order="F")
+    per = (ctypes.c_double * 3)(*period)
+    st = np.zeros(10)
+    r = lib().st_ic_shock(nx, ny, nz, ctypes.cast(per, _dp), mach, frac, width, k0, urms, gamma, seed, _p(Q), _p(st))
+    if r:
+        raise StError("st_ic_shock failed (%d)" % r)
+    return Q, st[:5].copy(), st[5:].copy()
 
 
 def ic_isotropic(nx, ny, nz, k0=4.0, urms=1.0, mt=0.3, gamma=1.4, seed=12345, period=(2 * np.pi,) * 3):
@@ -347,6 +365,12 @@ class Solver:
 
     def sync(self):
         self._chk(lib().st_sync(self.h), "st_sync")
+
+    def sponge(self, amp, power, frac, ref_left, ref_right):
+        """Sponge_Forcing in every stage (amp 0: off)."""
+        L = np.ascontiguousarray(ref_left, dtype=np.float64)
+        R = np.ascontiguousarray(ref_right, dtype=np.float64)
+        self._chk(lib().st_sponge(self.h, amp, power, frac, _p(L), _p(R)), "st_sponge")
 
     def autotune(self, reps=2):
         """Per-pass lane layout and tile width by timing; returns {pass: (narrow, nt)}."""

@@ -170,3 +170,34 @@ def test_np_steps_20(solvers):
     # only at the start, so its distance after 20 steps is a small multiple
     # of the single-perturbation spread
     assert e.max() < min(1e-8, max(1e-10, 10 * spread))
+
+
+def test_np_shock_sponge_steps():
+    """Config-2 pieces together: a Mach-1.5 shock with upstream turbulence,
+    Rankine-Hugoniot sponge layers at both ends, 10 steps on the GPU against
+    the oracle (tolerance from the oracle's own 1-ulp spread)."""
+    shape = (64, 16, 16)
+    Q, s1, s2 = st_amd.ic_shock(*shape, PER, mach=1.5, frac=0.25, urms=0.1, seed=3)
+    kw = dict(Re=300.0, hyper_param=1)
+    s = st_amd.Solver(*shape, period=PER, nonperiodic_x=1, **kw)
+    try:
+        s.sponge(5.0, 2.0, 0.4, s1, s2)
+        s.upload(Q)
+        t_g, st_g, _ = s.march(10)
+        Qg = s.download()
+    finally:
+        s.close()
+    outs = []
+    for pert in (0, 1):
+        o = O.Oracle(*shape, period=PER, nonper_x=1, **kw)
+        o.set_sponge(5.0, 2.0, 0.4, s1, s2)
+        Qi = Q if not pert else np.asfortranarray(
+            Q * (1 + np.random.default_rng(0).choice([-1, 1], size=Q.shape) * 2.2e-16))
+        outs.append(o.march(Qi, 10))
+    Qc, t_c = outs[0][0], outs[0][1]
+    spread = relerr(outs[1][0], Qc).max()
+    e = relerr(Qg, Qc)
+    _log("shock+sponge 10 steps relerr", e, "spread", spread)
+    assert st_g == 10 and abs(t_g - t_c) <= 1e-10 * t_c
+    assert e.max() < min(1e-8, max(1e-10, 10 * spread))
+    assert np.isfinite(Qg).all()
