@@ -43,14 +43,15 @@ PASS_STRUCT = {
 def kernel_symbol(kind, d, narrow, nt, n):
     """The kernel symbol (as rocprofv3 prints it) that runs pass kind.d at
     lines of n points (n a power of two, 16..256) with the given layout."""
-    if kind == "cross":
-        return "k_cross"
     M = 8 if n >= 64 else max(2, n // 8)
     P = n // M
     if narrow and 2 * P <= 64:
         M, P = M // 2, 2 * P
     if kind == "grad":
-        st = "PassGrad<1, true>" if nt == 1024 else ("PassGrad<3, false>" if d == "y" else "PassGrad<1, false>")
+        st = "PassGrad<1, true, false>" if nt == 1024 else (
+            "PassGrad<3, false, false>" if d == "y" else "PassGrad<1, false, false>")
+    elif kind == "grad_cross":
+        st = "PassGrad<1, true, true>" if nt == 1024 else "PassGrad<3, false, true>"
     else:
         st = PASS_STRUCT.get(kind)
         if st is None:

@@ -15,15 +15,16 @@
 
 namespace {
 
-constexpr int NKIND = 24;          // pass kinds + cross + reductions + fills
-constexpr int KIND_CROSS = 20, KIND_RED = 21, KIND_FILL = 22;
+constexpr int NKIND = 24;          // pass kinds + reductions + fills
+constexpr int KIND_RED = 21, KIND_FILL = 22;
+static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds");
 
 // Fields read + written per point by one launch of each kind (RMW counts
 // twice); the algorithmic bytes of a launch are this times 8 B per point.
 const char *kind_names[NKIND] = {
     "d1", "d2", "filter", "gauss", "grad", "mix_y", "mix_z", "mixh_x", "mixh_y", "mixh_z",
     "lap2", "lap2_eta", "gauss_eta", "gauss_coef", "hyper_div", "rk_filter_x", "filter_y",
-    "filter_z_fix", "", "", "cross", "reduce", "fill", ""};
+    "filter_z_fix", "grad_cross", "", "", "reduce", "fill", ""};
 
 struct Event {
     hipEvent_t a, b;
@@ -179,7 +180,10 @@ double kind_fields(int kind, const st::PassArgs &a)
     case st::K_RKF_X: return 15 + 5 + 5;
     case st::K_F_Y: return 10;
     case st::K_F_Z: return 10;
-    case KIND_CROSS: return 12 + 5 + 15 + 5 + (a.L2e ? 2 + 4 : 0);
+    // Q, G of y (write), G of x and z, the z and x partials, dQ; with the
+    // artificial transport the Laplacian of e (2 partials + the sum), |S|
+    // and C_beta
+    case st::K_GRAD_CROSS: return 5 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
     default: return 0;
     }
 }
@@ -522,9 +526,8 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
     const int hyp = c->p.hyper_param;
     st::PassArgs a = base;
     a.first = 1;
-    TRY(pass(c, st::K_GRAD, 1, a));
     // the z and x passes write their own partial sums (no read-modify-write);
-    // the cross kernel adds them up
+    // the y pass adds them up with its own terms and the viscous products
     {
         st::PassArgs b = a;
         b.dQ = c->dQp;
@@ -534,17 +537,8 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
         b.L2e = hyp ? c->L2ep + c->N : nullptr;
         TRY(pass(c, st::K_GRAD, 0, b));
     }
+    TRY(pass(c, st::K_GRAD_CROSS, 1, a));
     a.first = 0;
-    {
-        Event e;
-        if (c->prof) prof_begin(c, KIND_CROSS * 3, e);
-        hipError_t err = st::run_cross(a, hyp, c->s);
-        if (c->prof) {
-            prof_end(c, e);
-            c->prof_fields[KIND_CROSS * 3] = kind_fields(KIND_CROSS, a);
-        }
-        if (err != hipSuccess) return fail(c, -4, "cross: %s", hipGetErrorString(err));
-    }
     if (hyp) {
         a.first = 1;
         TRY(pass(c, st::K_MIXH_X, 0, a));
@@ -1051,7 +1045,7 @@ extern "C" int st_autotune(st_ctx *c, int reps)
     int rc = 0;
     for (int ci = 0; ci < 6 && !rc; ci++) {
         int valid[NKIND][3] = {};
-        for (int k = st::K_GRAD; k <= st::K_F_Z; k++)
+        for (int k = st::K_GRAD; k < st::K_NPASS; k++)
             for (int d = 0; d < 3; d++) {
                 c->cfg_narrow[k][d] = cand[ci][0];
                 c->cfg_nt[k][d] = cand[ci][1];
@@ -1068,7 +1062,7 @@ extern "C" int st_autotune(st_ctx *c, int reps)
         for (int r = 0; r < reps && !rc; r++) rc = seq_stage(c, base, ra[0], rb[0], 1.0e-6);
         if (hipStreamSynchronize(c->s) != hipSuccess && !rc) rc = fail(c, -3, "autotune: stream error");
         prof_collect(c);
-        for (int k = st::K_GRAD; k <= st::K_F_Z; k++)
+        for (int k = st::K_GRAD; k < st::K_NPASS; k++)
             for (int d = 0; d < 3; d++) {
                 const int pk = k * 3 + d;
                 if (!valid[k][d] || !c->prof_cnt[pk]) continue;

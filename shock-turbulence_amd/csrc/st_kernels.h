@@ -55,6 +55,8 @@ enum PassKind {
     K_GAUSS_ETA, K_GAUSS_ETA_LAST,
     K_HDIV,
     K_RKF_X, K_F_Y, K_F_Z,
+    K_GRAD_CROSS,
+    K_NPASS,              // pass kinds end here
 };
 
 // Default launch layout of a pass kind: narrow = 1 uses the tables with half
@@ -67,7 +69,6 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
 hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
-hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s);
 
 }  // namespace st
 

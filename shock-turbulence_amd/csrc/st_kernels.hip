@@ -14,63 +14,6 @@
 
 namespace st {
 
-// ---------------------------------------------------------------------
-// pointwise kernels
-// ---------------------------------------------------------------------
-
-// Viscous terms built from products of first derivatives
-// (src/rhsv_periodic_sf.f:1128-1154 without the mixed and own-direction
-// second derivatives, and :1252-1275), plus |S| (src/hyper_dynamic.f:12-33,
-// 227-245) and the dilatation sensor C_beta (src/hyper_dynamic.f:142-160;
-// skipped when Cb is null: the constant-coefficient src/hyper_routines.f).
-__global__ void __launch_bounds__(256) k_cross(PassArgs a0, int hyp)
-{
-    const int slot = a0.slots ? a0.slots[blockIdx.y] : blockIdx.y;
-    if (a0.act && !a0.act[slot]) return;
-    PassArgs a;
-    slot_params(a0, slot, a);
-    const long N = a.N, off = (long)slot * a.ni;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
-        const long gi = off + i;
-        const Phys &ph = a.ph;
-        const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
-        const Prim pr = prim(ph, q1, q2, q3, q4, q5);
-        const double *G = a.G;
-        const double gxx = G[gi], gyx = G[N + gi], gzx = G[2 * N + gi];
-        const double gxy = G[3 * N + gi], gyy = G[4 * N + gi], gzy = G[5 * N + gi];
-        const double gxz = G[6 * N + gi], gyz = G[7 * N + gi], gzz = G[8 * N + gi];
-        const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
-        const double tx = mup * G[9 * N + gi], ty = mup * G[10 * N + gi], tz = mup * G[11 * N + gi];
-        const double f1 = ph.bulk - 2.0 / 3.0;
-        const double dil = gxx + gyy + gzz;
-        const double t1 = gyx + gxy, t2 = gzx + gxz, t3 = gzy + gyz;
-        const double fX = tx * (2.0 * gxx + f1 * dil) + ty * t1 + tz * t2;
-        const double fY = ty * (2.0 * gyy + f1 * dil) + tx * t1 + tz * t3;
-        const double fZ = tz * (2.0 * gzz + f1 * dil) + tx * t2 + ty * t3;
-        // the y pass left its terms in dQ, the z and x passes theirs in dQp
-        const double *Pz = a.dQp, *Px = a.dQp + 5 * N;
-        a.dQ[gi] = ((a.dQ[gi] + Pz[gi]) + Px[gi]) + fX;
-        a.dQ[N + gi] = ((a.dQ[N + gi] + Pz[N + gi]) + Px[N + gi]) + fY;
-        a.dQ[2 * N + gi] = ((a.dQ[2 * N + gi] + Pz[2 * N + gi]) + Px[2 * N + gi]) + fZ;
-        a.dQ[3 * N + gi] = (a.dQ[3 * N + gi] + Pz[3 * N + gi]) + Px[3 * N + gi];
-        a.dQ[4 * N + gi] = ((a.dQ[4 * N + gi] + Pz[4 * N + gi]) + Px[4 * N + gi])
-                           + (pr.u * fX + pr.v * fY + pr.w * fZ
-                              + mu * (2.0 * (gxx * gxx + gyy * gyy + gzz * gzz) + f1 * dil * dil)
-                              + mu * (t1 * t1 + t2 * t2 + t3 * t3));
-        if (a.L2e) a.L2e[gi] = (a.L2e[gi] + a.L2ep[gi]) + a.L2ep[N + gi];
-        if (hyp) {
-            const double s12 = 0.5 * (gxy + gyx), s13 = 0.5 * (gxz + gzx), s23 = 0.5 * (gyz + gzy);
-            a.S[gi] = sqrt(0.5 * (gxx * gxx + gyy * gyy + gzz * gzz + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
-            if (a.Cb) {
-                const double wx = gzy - gyz, wy = gxz - gzx, wz = gyx - gxy;
-                const double om2 = wx * wx + wy * wy + wz * wz;
-                const double fac = 0.5 * (1.0 - tanh(2.5 + 10.0 * dil));
-                a.Cb[gi] = fac * dil * dil / (dil * dil + om2 + 1.0e-7) + 1.0e-20;
-            }
-        }
-    }
-}
-
 void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
 {
     // Chosen by st_autotune at 256^3 (per pass and direction): wider tiles
@@ -81,6 +24,7 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     C c = {0, 256};
     switch (kind) {
     case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_GRAD_CROSS: c = {1, 256}; break;
     case K_MIX_Y: c = {1, 256}; break;
     case K_MIX_Z: c = {1, 512}; break;
     case K_MIXH_X: c = {0, 512}; break;
@@ -105,7 +49,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
     switch (kind) {
     case K_D1: case K_D2: case K_FILT: case K_GAUSS:
         return launch_ops(kind, t, nt, dir, My, Mz, Mx, a, s);
-    case K_GRAD:
+    case K_GRAD: case K_GRAD_CROSS:
         return launch_grad(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_MIX_Y: case K_MIX_Z: case K_MIXH_X: case K_MIXH_Y: case K_MIXH_Z:
         return launch_mix(kind, t, nt, dir, My, Mz, Mx, a, s);
@@ -115,14 +59,6 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
         return launch_filter(kind, t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
-}
-
-hipError_t run_cross(const PassArgs &a, int hyp, hipStream_t s)
-{
-    long blocks = (a.ni + 255) / 256;
-    if (blocks > 2048 * 8) blocks = 2048 * 8;
-    hipLaunchKernelGGL(k_cross, dim3((unsigned)blocks, (unsigned)a.nslot), dim3(256), 0, s, a, hyp);
-    return hipGetLastError();
 }
 
 }  // namespace st

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "st_engine.hpp"
 #include "st_kernels.h"
 
@@ -14,6 +16,13 @@ namespace st {
 // ---------------------------------------------------------------------
 template <class PASS>
 constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PASS::NEARLY; }
+
+// NCARRY of a pass (0 when it names none): values per point that
+// store_early() hands to store() in registers (REUSE passes only).
+template <class T, class = void>
+struct carry_n { static constexpr int value = 0; };
+template <class T>
+struct carry_n<T, std::void_t<decltype(T::NCARRY)>> { static constexpr int value = T::NCARRY; };
 
 // Slots 0..NSLOT-1 hold the staged inputs; compute() may write final
 // results early (freeing their registers) into slots NSLOT..NSLOT+NEARLY-1,
@@ -27,6 +36,9 @@ constexpr int pass_slots() { return PASS::REUSE ? PASS::NSLOT : PASS::NSLOT + PA
 // (read-modify-write operands): prefetch() loads them during the load phase,
 // which visits the same points per thread as the store phase, and they
 // reach store_early() (REUSE) or store() appended after the results.
+// A REUSE pass with NCARRY > 0 gets a per-point register array in both
+// store phases (same points per thread): store_early() fills it, store()
+// reads it, so early results can feed the late store without a slot.
 template <int M, int P, int NT, bool NP, class PASS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
 k_pass(Geo g, PassArgs a0)
@@ -40,6 +52,9 @@ k_pass(Geo g, PassArgs a0)
     constexpr int NPF = PASS::NPF;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
+    constexpr int NCR = carry_n<PASS>::value;
+    static_assert(NCR == 0 || (RE && NO > 0), "a carry goes from store_early to store");
+    double cr[M][NCR > 0 ? NCR : 1];
     const long tile = tile_of(g);
     const int si = (int)(tile / g.tpi);
     const int slot = a0.slots ? a0.slots[si] : si;
@@ -82,7 +97,8 @@ k_pass(Geo g, PassArgs a0)
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
 #pragma unroll
             for (int j = 0; j < NPF; j++) v[NE + j] = pf[r][j];
-            PASS::store_early(g, a, gi, v);
+            if constexpr (NCR > 0) PASS::store_early(g, a, gi, v, cr[r]);
+            else PASS::store_early(g, a, gi, v);
         }
         if constexpr (NO == 0) return;
         __syncthreads();
@@ -109,7 +125,8 @@ k_pass(Geo g, PassArgs a0)
         }
 #pragma unroll
         for (int j = 0; j < NPF; j++) v[NV + j] = pf[r][j];
-        PASS::store(g, a, gi, v);
+        if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
+        else PASS::store(g, a, gi, v);
     }
 }
 

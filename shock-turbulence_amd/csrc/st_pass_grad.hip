@@ -24,10 +24,17 @@ namespace st {
 // 3 suits the unit-stride y lines, the strided x and z lines run faster with
 // the compiler's own choice.  LEAN selects the operator order that keeps
 // fewer values live (for 1024-thread tiles, which allow 128 registers).
-template <int MW, bool LEAN>
+// CROSS (y lines, after the z and x passes have left their partial sums):
+// the store phase also assembles the right-hand side of the first-derivative
+// part -- the y terms, the z and x partials and the viscous terms built from
+// products of first derivatives -- and |S| and C_beta, from the derivatives
+// of all three directions at the point (the y ones carried from the early
+// store in registers).
+template <int MW, bool LEAN, bool CROSS = false>
 struct PassGrad {
     static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
+    static constexpr int NCARRY = CROSS ? 4 : 0;
     static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
@@ -263,6 +270,63 @@ struct PassGrad {
 #pragma unroll
         for (int c = 0; c < 5; c++) upd(&a.dQ[c * a.N + gi], v[c], a.first);
     }
+
+    // CROSS: G(., y) and dT/dy go out and stay in the carry; the Laplacian
+    // of e adds the z and x partials in the order the passes ran.
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v, double *cr)
+    {
+        const long N = a.N;
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.G[(3 + c) * N + gi] = v[c];
+        a.G[10 * N + gi] = v[3];
+#pragma unroll
+        for (int c = 0; c < 4; c++) cr[c] = v[c];
+        if (a.L2e) a.L2e[gi] = (v[4] + a.L2ep[gi]) + a.L2ep[N + gi];
+    }
+    // CROSS: viscous terms built from products of first derivatives
+    // (src/rhsv_periodic_sf.f:1128-1154 without the mixed and own-direction
+    // second derivatives, and :1252-1275), plus |S| (src/hyper_dynamic.f:
+    // 12-33, 227-245) and the dilatation sensor C_beta
+    // (src/hyper_dynamic.f:142-160; skipped when Cb is null: the
+    // constant-coefficient src/hyper_routines.f); |S| and C_beta only with
+    // the artificial transport on (L2e set).
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v, const double *cr)
+    {
+        const long N = a.N;
+        const Phys &ph = a.ph;
+        const Prim pr = prim(ph, a.Q[gi], a.Q[N + gi], a.Q[2 * N + gi], a.Q[3 * N + gi], a.Q[4 * N + gi]);
+        const double *G = a.G;
+        const double gxx = G[gi], gyx = G[N + gi], gzx = G[2 * N + gi];
+        const double gxy = cr[0], gyy = cr[1], gzy = cr[2];
+        const double gxz = G[6 * N + gi], gyz = G[7 * N + gi], gzz = G[8 * N + gi];
+        const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
+        const double tx = mup * G[9 * N + gi], ty = mup * cr[3], tz = mup * G[11 * N + gi];
+        const double f1 = ph.bulk - 2.0 / 3.0;
+        const double dil = gxx + gyy + gzz;
+        const double t1 = gyx + gxy, t2 = gzx + gxz, t3 = gzy + gyz;
+        const double fX = tx * (2.0 * gxx + f1 * dil) + ty * t1 + tz * t2;
+        const double fY = ty * (2.0 * gyy + f1 * dil) + tx * t1 + tz * t3;
+        const double fZ = tz * (2.0 * gzz + f1 * dil) + tx * t2 + ty * t3;
+        const double *Pz = a.dQp, *Px = a.dQp + 5 * N;
+        a.dQ[gi] = ((v[0] + Pz[gi]) + Px[gi]) + fX;
+        a.dQ[N + gi] = ((v[1] + Pz[N + gi]) + Px[N + gi]) + fY;
+        a.dQ[2 * N + gi] = ((v[2] + Pz[2 * N + gi]) + Px[2 * N + gi]) + fZ;
+        a.dQ[3 * N + gi] = (v[3] + Pz[3 * N + gi]) + Px[3 * N + gi];
+        a.dQ[4 * N + gi] = ((v[4] + Pz[4 * N + gi]) + Px[4 * N + gi])
+                           + (pr.u * fX + pr.v * fY + pr.w * fZ
+                              + mu * (2.0 * (gxx * gxx + gyy * gyy + gzz * gzz) + f1 * dil * dil)
+                              + mu * (t1 * t1 + t2 * t2 + t3 * t3));
+        if (a.L2e) {
+            const double s12 = 0.5 * (gxy + gyx), s13 = 0.5 * (gxz + gzx), s23 = 0.5 * (gyz + gzy);
+            a.S[gi] = sqrt(0.5 * (gxx * gxx + gyy * gyy + gzz * gzz + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+            if (a.Cb) {
+                const double wx = gzy - gyz, wy = gxz - gzx, wz = gyx - gxy;
+                const double om2 = wx * wx + wy * wy + wz * wz;
+                const double fac = 0.5 * (1.0 - tanh(2.5 + 10.0 * dil));
+                a.Cb[gi] = fac * dil * dil / (dil * dil + om2 + 1.0e-7) + 1.0e-20;
+            }
+        }
+    }
 };
 
 
@@ -273,6 +337,10 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
         if (nt == 1024) return launch<PassGrad<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
         if (dir == 1) return launch<PassGrad<3, false>, false>(t, nt, dir, My, Mz, Mx, a, s);
         return launch<PassGrad<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GRAD_CROSS:
+        if (dir != 1) return hipErrorInvalidValue;
+        if (nt == 1024) return launch<PassGrad<1, true, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<3, false, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }
