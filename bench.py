@@ -36,7 +36,7 @@ PASS_STRUCT = {
     "mixh_y": "PassMix<1, true>", "mixh_z": "PassMix<2, true>", "lap2": "PassLap2<false>",
     "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta<false>", "gauss_coef": "PassGaussEta<true>",
     "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>", "filter_y": "PassFilterQ<1>",
-    "filter_z_fix": "PassFilterQ<2>",
+    "filter_z_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
 }
 
 

@@ -24,7 +24,7 @@ static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds
 const char *kind_names[NKIND] = {
     "d1", "d2", "filter", "gauss", "grad", "mix_y", "mix_z", "mixh_x", "mixh_y", "mixh_z",
     "lap2", "lap2_eta", "gauss_eta", "gauss_coef", "hyper_div", "rk_filter_x", "filter_y",
-    "filter_z_fix", "grad_cross", "", "", "reduce", "fill", ""};
+    "filter_z_fix", "grad_cross", "hdiv_rkf", "", "reduce", "fill", ""};
 
 struct Event {
     hipEvent_t a, b;
@@ -183,6 +183,8 @@ double kind_fields(int kind, const st::PassArgs &a)
     // Q, G of y (write), G of x and z, the z and x partials, dQ; with the
     // artificial transport the Laplacian of e (2 partials + the sum), |S|
     // and C_beta
+    // the divergence inputs (Hc, 8 G, Q), dQ, QW; QW and Q written
+    case st::K_HDIV_RKF: return 3 + 8 + 5 + 5 + 5 + 5 + 5;
     case st::K_GRAD_CROSS: return 5 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
     default: return 0;
     }
@@ -552,8 +554,10 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
     return 0;
 }
 
-// Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ.
-int seq_hyper(st_ctx *c, const st::PassArgs &base)
+// Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ.  With rk set, the last
+// (x) divergence pass also applies the stage update and the x filter
+// (K_HDIV_RKF), so the stage goes on with the y and z filters.
+int seq_hyper(st_ctx *c, const st::PassArgs &base, const st::PassArgs *rk = nullptr)
 {
     if (!c->p.hyper_param) return 0;
     st::PassArgs a = base;
@@ -567,6 +571,7 @@ int seq_hyper(st_ctx *c, const st::PassArgs &base)
     TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));
     TRY(pass(c, st::K_HDIV, 1, a));
     TRY(pass(c, st::K_HDIV, 2, a));
+    if (rk) return pass(c, st::K_HDIV_RKF, 0, *rk);
     TRY(pass(c, st::K_HDIV, 0, a));
     return 0;
 }
@@ -574,17 +579,20 @@ int seq_hyper(st_ctx *c, const st::PassArgs &base)
 // One stage body (src/lddrk4_periodic_sf.f:344-356).
 int seq_stage(st_ctx *c, const st::PassArgs &base, double ra, double rb, double dt)
 {
-    TRY(seq_rhs(c, base));
-    TRY(seq_hyper(c, base));
-    if (c->spg) {                                            // the hook order of src/lddrk4_nonperiodic_mf.f:462-466
-        const hipError_t e = st::run_sponge(base, c->spg, c->Mx, c->spg_left, c->s);
-        if (e != hipSuccess) return fail(c, -4, "sponge: %s", hipGetErrorString(e));
-    }
     st::PassArgs a = base;
     a.rk_a = ra;
     a.rk_b = rb;
     a.dt = dt;
-    TRY(pass(c, st::K_RKF_X, 0, a));
+    TRY(seq_rhs(c, base));
+    // the update joins the last divergence pass unless the sponge comes
+    // between them
+    const bool fuse = c->p.hyper_param && !c->spg && !getenv("ST_NO_FUSE_RK");
+    TRY(seq_hyper(c, base, fuse ? &a : nullptr));
+    if (c->spg) {                                            // the hook order of src/lddrk4_nonperiodic_mf.f:462-466
+        const hipError_t e = st::run_sponge(base, c->spg, c->Mx, c->spg_left, c->s);
+        if (e != hipSuccess) return fail(c, -4, "sponge: %s", hipGetErrorString(e));
+    }
+    if (!fuse) TRY(pass(c, st::K_RKF_X, 0, a));
     TRY(pass(c, st::K_F_Y, 1, a));
     a.fix = 1;
     TRY(pass(c, st::K_F_Z, 2, a));

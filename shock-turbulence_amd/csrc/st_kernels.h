@@ -56,6 +56,7 @@ enum PassKind {
     K_HDIV,
     K_RKF_X, K_F_Y, K_F_Z,
     K_GRAD_CROSS,
+    K_HDIV_RKF,
     K_NPASS,              // pass kinds end here
 };
 
@@ -68,6 +69,7 @@ hipError_t launch_ops(int kind, const DirTab &t, int nt, int dir, int My, int Mz
 hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_hdivrk(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 
 }  // namespace st

@@ -36,6 +36,7 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     case K_GAUSS_ETA_LAST: c = {1, 1024}; break;
     case K_HDIV: c = dir == 2 ? C{1, 1024} : C{0, 512}; break;
     case K_RKF_X: c = {0, 512}; break;
+    case K_HDIV_RKF: c = {0, 512}; break;
     case K_F_Y: c = {0, 256}; break;
     case K_F_Z: c = {0, 256}; break;
     default: break;
@@ -57,6 +58,8 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
         return launch_hyper(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_RKF_X: case K_F_Y: case K_F_Z:
         return launch_filter(kind, t, nt, dir, My, Mz, Mx, a, s);
+    case K_HDIV_RKF:
+        return launch_hdivrk(kind, t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -24,6 +24,15 @@ struct carry_n { static constexpr int value = 0; };
 template <class T>
 struct carry_n<T, std::void_t<decltype(T::NCARRY)>> { static constexpr int value = T::NCARRY; };
 
+// MID of a pass (false when it names none): a second compute phase.  After
+// compute(), mid() rewrites the NSLOT staged values of each point in memory
+// order (it may read and write global fields of its own points), then
+// compute2() runs on the rewritten slots in the line mapping.
+template <class T, class = void>
+struct has_mid { static constexpr bool value = false; };
+template <class T>
+struct has_mid<T, std::void_t<decltype(T::MID)>> { static constexpr bool value = T::MID; };
+
 // Slots 0..NSLOT-1 hold the staged inputs; compute() may write final
 // results early (freeing their registers) into slots NSLOT..NSLOT+NEARLY-1,
 // or, with REUSE, over its own points of the input slots 0..NEARLY-1 once
@@ -83,6 +92,24 @@ k_pass(Geo g, PassArgs a0)
     const Lane<M, P, NP> L = make_lane<M, P, NP>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
     PASS::template compute<M, P, NP>(g, a, L, out);
+    if constexpr (has_mid<PASS>::value) {
+        static_assert(RE && NO == 0, "a second compute phase works on the slots");
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            int b, s;
+            const long gi = W.at(g, r, b, s);
+            const int li = lds_idx<M>(b, s % M, s / M, P);
+            double v[NS];
+#pragma unroll
+            for (int q = 0; q < NS; q++) v[q] = lds[q * ss + li];
+            PASS::mid(g, a, gi, v);
+#pragma unroll
+            for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
+        }
+        __syncthreads();
+        PASS::template compute2<M, P, NP>(g, a, L);
+    }
 
     // 3. store (through LDS, memory order)
     __syncthreads();

@@ -23,9 +23,11 @@ struct PassFilterQ {
         for (int c = 0; c < 5; c++) {
             const long o = c * a.N + gi;
             if (KIND == 0) {
-                const double qw = a.rk_a * a.QW[o] + a.dt * a.dQ[o];
+                // contractions written out, so that the fused form of this
+                // update (PassHdivRkf) rounds alike
+                const double qw = fma(a.dt, a.dQ[o], a.rk_a * a.QW[o]);
                 a.QW[o] = qw;
-                v[c] = a.Q[o] + a.rk_b * qw;
+                v[c] = fma(a.rk_b, qw, a.Q[o]);
             } else {
                 v[c] = a.Q[o];
             }
