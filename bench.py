@@ -34,7 +34,7 @@ STAGES_PER_STEP = 11
 PASS_STRUCT = {
     "mix_y": "PassMix<1, false>", "mix_z": "PassMix<2, false>", "mixh_x": "PassMix<0, true>",
     "mixh_y": "PassMix<1, true>", "mixh_z": "PassMix<2, true>", "lap2": "PassLap2<false>",
-    "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta<false>", "gauss_coef": "PassGaussEta<true>",
+    "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta", "gauss_hdiv": "PassGaussHdiv",
     "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>", "filter_y": "PassFilterQ<1>",
     "filter_z_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
 }

@@ -23,7 +23,7 @@ static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds
 // twice); the algorithmic bytes of a launch are this times 8 B per point.
 const char *kind_names[NKIND] = {
     "d1", "d2", "filter", "gauss", "grad", "mix_y", "mix_z", "mixh_x", "mixh_y", "mixh_z",
-    "lap2", "lap2_eta", "gauss_eta", "gauss_coef", "hyper_div", "rk_filter_x", "filter_y",
+    "lap2", "lap2_eta", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter_x", "filter_y",
     "filter_z_fix", "grad_cross", "hdiv_rkf", "", "reduce", "fill", ""};
 
 struct Event {
@@ -175,7 +175,8 @@ double kind_fields(int kind, const st::PassArgs &a)
     case st::K_LAP2: return a.first ? 2 : 3;
     case st::K_LAP2_LAST: return 1 + 1 + 1 + 5 + 2;
     case st::K_GAUSS_ETA: return 4;
-    case st::K_GAUSS_ETA_LAST: return 2 + 1 + 3;
+    // eta, C_beta, beta* mu* kappa* (written), the z-flux inputs (8 G, 4 Q), dQ 4 (read and written)
+    case st::K_GAUSS_ETA_LAST: return 2 + (a.Cb ? 1 : 0) + 3 + 8 + 4 + 8;
     case st::K_HDIV: return 3 + 8 + 4 + 8;
     case st::K_RKF_X: return 15 + 5 + 5;
     case st::K_F_Y: return 10;
@@ -568,9 +569,8 @@ int seq_hyper(st_ctx *c, const st::PassArgs &base, const st::PassArgs *rk = null
     TRY(pass(c, st::K_LAP2_LAST, 2, a));
     TRY(pass(c, st::K_GAUSS_ETA, 0, a));
     TRY(pass(c, st::K_GAUSS_ETA, 1, a));
-    TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));
+    TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));            // and the z divergence
     TRY(pass(c, st::K_HDIV, 1, a));
-    TRY(pass(c, st::K_HDIV, 2, a));
     if (rk) return pass(c, st::K_HDIV_RKF, 0, *rk);
     TRY(pass(c, st::K_HDIV, 0, a));
     return 0;

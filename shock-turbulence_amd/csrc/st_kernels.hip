@@ -28,13 +28,13 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     case K_MIX_Y: c = {1, 256}; break;
     case K_MIX_Z: c = {1, 512}; break;
     case K_MIXH_X: c = {0, 512}; break;
-    case K_MIXH_Y: c = {1, 512}; break;
+    case K_MIXH_Y: c = {1, 256}; break;
     case K_MIXH_Z: c = {1, 512}; break;
-    case K_LAP2: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_LAP2: c = dir == 0 ? C{0, 512} : C{0, 1024}; break;
     case K_LAP2_LAST: c = {0, 1024}; break;
-    case K_GAUSS_ETA: c = dir == 0 ? C{0, 256} : C{0, 512}; break;
-    case K_GAUSS_ETA_LAST: c = {1, 1024}; break;
-    case K_HDIV: c = dir == 2 ? C{1, 1024} : C{0, 512}; break;
+    case K_GAUSS_ETA: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
+    case K_GAUSS_ETA_LAST: c = {0, 512}; break;
+    case K_HDIV: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
     case K_RKF_X: c = {0, 512}; break;
     case K_HDIV_RKF: c = {0, 512}; break;
     case K_F_Y: c = {0, 256}; break;

@@ -30,8 +30,14 @@ struct PassHyperDiv {
     DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
     {
         const long N = a.N;
+        fluxes(g, a, gi, a.Hc[gi], a.Hc[N + gi], a.Hc[2 * N + gi], v);
+    }
+    // the staged fluxes of direction g.dir from the coefficients at the point
+    DEV static void fluxes(const Geo &g, const PassArgs &a, long gi, double beta, double mu, double kap,
+                           double (&v)[4])
+    {
+        const long N = a.N;
         const double *G = a.G;
-        const double beta = a.Hc[gi], mu = a.Hc[N + gi], kap = a.Hc[2 * N + gi];
         const double gxx = G[gi], gyy = G[4 * N + gi], gzz = G[8 * N + gi];
         const double dil = gxx + gyy + gzz;
         const double q4 = a.Q[3 * N + gi];

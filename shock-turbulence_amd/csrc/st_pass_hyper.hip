@@ -39,10 +39,8 @@ struct PassLap2 : NoReuse {
 };
 
 // ---------------------------------------------------------------------
-// Gaussian smoothing of eta (in place).  The z pass (last) forms the
-// transport coefficients beta*, mu*, kappa* (src/hyper_dynamic.f:303-314).
+// Gaussian smoothing of eta (in place), x and y passes.
 // ---------------------------------------------------------------------
-template <bool LAST>
 struct PassGaussEta : NoReuse {
     static constexpr int NSLOT = 2, NOUT = 2, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[2])
@@ -64,26 +62,84 @@ struct PassGaussEta : NoReuse {
     }
     DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v)
     {
-        if (!LAST) {
-            a.eta[gi] = v[0];
-            a.eta[a.N + gi] = v[1];
-            return;
-        }
-        const Phys &ph = a.ph;
-        // beta*; constant C_beta = 1 without the sensor (src/hyper_routines.f:99-101,204)
-        a.Hc[gi] = (a.Cb ? a.Cb[gi] : 1.0) * v[0] * ph.dx6;
-        a.Hc[a.N + gi] = ph.cmu * v[0] * ph.dx6;        // mu*
-        a.Hc[2 * a.N + gi] = ph.ck * v[1] * ph.dx5;     // kappa*
+        a.eta[gi] = v[0];
+        a.eta[a.N + gi] = v[1];
     }
 };
+
+// ---------------------------------------------------------------------
+// The z pass of the Gaussian smoothing fused with the z pass of the
+// divergence: the smoothed eta gives the transport coefficients beta*,
+// mu*, kappa* (src/hyper_dynamic.f:303-314), which are written out for the
+// other divergence passes and, at the same point, give the z fluxes of the
+// divergence (src/hyper_rhs.f:159-491), whose D1z is added into dQ.
+//   compute   Gaussian along z of the two eta fields
+//   mid       per point: beta*, mu*, kappa* (written), the staged z fluxes
+//   compute2  D1z of the fluxes
+// ---------------------------------------------------------------------
+struct PassGaussHdiv {
+    static constexpr int NSLOT = 4, NOUT = 0, NEARLY = 4;
+    static constexpr bool REUSE = true;
+    static constexpr bool MID = true;
+    static constexpr int MIN_WAVES = 1;
+    static constexpr int LOAD_UNROLL = 8;
+    static constexpr int NPF = PassHyperDiv::NPF;
+    DEV static void prefetch(const Geo &g, const PassArgs &a, long gi, double *p) { PassHyperDiv::prefetch(g, a, gi, p); }
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[4])
+    {
+        v[0] = a.eta[gi];
+        v[1] = a.eta[a.N + gi];
+        v[2] = 0.0;
+        v[3] = 0.0;
+    }
+    template <int M, int P, bool NP>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    {
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(m, k);
+            window<M>(f, L, w);
+            op_gauss<M>(w, a.C, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) L.st(m, k, r[k]);
+        }
+    }
+    DEV static void mid(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
+    {
+        const Phys &ph = a.ph;
+        const long N = a.N;
+        // beta*; constant C_beta = 1 without the sensor (src/hyper_routines.f:99-101,204)
+        const double beta = (a.Cb ? a.Cb[gi] : 1.0) * v[0] * ph.dx6;
+        const double mu = ph.cmu * v[0] * ph.dx6;
+        const double kap = ph.ck * v[1] * ph.dx5;
+        a.Hc[gi] = beta;
+        a.Hc[N + gi] = mu;
+        a.Hc[2 * N + gi] = kap;
+        PassHyperDiv::fluxes(g, a, gi, beta, mu, kap, v);
+    }
+    template <int M, int P, bool NP>
+    DEV static void compute2(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L)
+    {
+        double o[1][M];
+        PassHyperDiv::compute<M, P, NP>(g, a, L, o);
+    }
+    DEV static void store(const Geo &, const PassArgs &, long, const double *) {}
+    DEV static void store_early(const Geo &g, const PassArgs &a, long gi, const double *v)
+    {
+        PassHyperDiv::store_early(g, a, gi, v);
+    }
+};
+
 
 hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     switch (kind) {
     case K_LAP2: return launch<PassLap2<false>>(t, nt, dir, My, Mz, Mx, a, s);
     case K_LAP2_LAST: return launch<PassLap2<true>, false>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_GAUSS_ETA: return launch<PassGaussEta<false>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_GAUSS_ETA_LAST: return launch<PassGaussEta<true>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA: return launch<PassGaussEta>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_GAUSS_ETA_LAST: return launch<PassGaussHdiv, false>(t, nt, dir, My, Mz, Mx, a, s);
     case K_HDIV: return launch<PassHyperDiv>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
