@@ -32,11 +32,9 @@ STAGES_PER_STEP = 11
 
 # kernel template of each pass kind (csrc/st_pass_*.hip)
 PASS_STRUCT = {
-    "mix_y": "PassMix<1, false>", "mix_z": "PassMix<2, false>", "mixh_x": "PassMix<0, true>",
-    "mixh_y": "PassMix<1, true>", "mixh_z": "PassMix<2, true>", "lap2": "PassLap2<false>",
-    "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta", "gauss_hdiv": "PassGaussHdiv",
-    "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>", "filter_y": "PassFilterQ<1>",
-    "filter_z_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
+    "lap2": "PassLap2<false>", "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta",
+    "gauss_hdiv": "PassGaussHdiv", "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>",
+    "filter_y": "PassFilterQ<1>", "filter_z_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
 }
 
 
@@ -48,10 +46,12 @@ def kernel_symbol(kind, d, narrow, nt, n):
     if narrow and 2 * P <= 64:
         M, P = M // 2, 2 * P
     if kind == "grad":
-        st = "PassGrad<1, true, false>" if nt == 1024 else (
-            "PassGrad<3, false, false>" if d == "y" else "PassGrad<1, false, false>")
+        mx = 2 if d == "z" else 0
+        st = "PassGrad<1, %s, false, %d>" % ("true" if nt == 1024 else "false", mx)
     elif kind == "grad_cross":
-        st = "PassGrad<1, true, true>" if nt == 1024 else "PassGrad<3, false, true>"
+        st = "PassGrad<1, true, true, 4>" if nt == 1024 else "PassGrad<2, false, true, 4>"
+    elif kind == "lap_se":
+        st = "PassLapSE<%s>" % ("false" if d == "x" else "true")
     else:
         st = PASS_STRUCT.get(kind)
         if st is None:

@@ -22,9 +22,9 @@ static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds
 // Fields read + written per point by one launch of each kind (RMW counts
 // twice); the algorithmic bytes of a launch are this times 8 B per point.
 const char *kind_names[NKIND] = {
-    "d1", "d2", "filter", "gauss", "grad", "mix_y", "mix_z", "mixh_x", "mixh_y", "mixh_z",
+    "d1", "d2", "filter", "gauss", "grad", "lap_se",
     "lap2", "lap2_eta", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter_x", "filter_y",
-    "filter_z_fix", "grad_cross", "hdiv_rkf", "", "reduce", "fill", ""};
+    "filter_z_fix", "grad_cross", "hdiv_rkf", "", "", "", "", "", "reduce", "fill", ""};
 
 struct Event {
     hipEvent_t a, b;
@@ -159,19 +159,17 @@ st::PassArgs base_args(st_ctx *c)
 }
 
 // fields read+written per point for one launch of a pass kind
-double kind_fields(int kind, const st::PassArgs &a)
+double kind_fields(int kind, int dir, const st::PassArgs &a)
 {
     switch (kind) {
     case st::K_D1: return a.iadd ? 3 : 2;
     case st::K_D2: return a.iadd ? 3 : 2;
     case st::K_FILT: return 2;
     case st::K_GAUSS: return 2;
-    case st::K_GRAD: return 5 + 4 + (a.first ? 5 : 10) + (a.L2e ? (a.first ? 1 : 2) : 0);
-    case st::K_MIX_Y: return 4 + 5 + 8;
-    case st::K_MIX_Z: return 2 + 5 + 6;
-    case st::K_MIXH_X: return 2 + 2;
-    case st::K_MIXH_Y: return 6 + 5 + 8 + 4;
-    case st::K_MIXH_Z: return 4 + 5 + 6 + 4;
+    // Q, G of the direction, the dQ partial, the Lap e partial; z also reads
+    // the two stored derivatives of its mixed terms
+    case st::K_GRAD: return 5 + 4 + 5 + (a.L2e ? 1 : 0) + (dir == 2 ? 2 : 0);
+    case st::K_LAPSE: return a.first ? 2 + 2 : 2 + 4;
     case st::K_LAP2: return a.first ? 2 : 3;
     case st::K_LAP2_LAST: return 1 + 1 + 1 + 5 + 2;
     case st::K_GAUSS_ETA: return 4;
@@ -181,12 +179,13 @@ double kind_fields(int kind, const st::PassArgs &a)
     case st::K_RKF_X: return 15 + 5 + 5;
     case st::K_F_Y: return 10;
     case st::K_F_Z: return 10;
-    // Q, G of y (write), G of x and z, the z and x partials, dQ; with the
+    // Q, G of y (write), the 4 inputs of the mixed terms, G of x and z (8),
+    // the z and x partials, dQ; with the
     // artificial transport the Laplacian of e (2 partials + the sum), |S|
     // and C_beta
     // the divergence inputs (Hc, 8 G, Q), dQ, QW; QW and Q written
     case st::K_HDIV_RKF: return 3 + 8 + 5 + 5 + 5 + 5 + 5;
-    case st::K_GRAD_CROSS: return 5 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
+    case st::K_GRAD_CROSS: return 5 + 4 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
     default: return 0;
     }
 }
@@ -211,7 +210,7 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
     if (a.dry) return err == hipSuccess ? 0 : -4;
     if (prof) {
         prof_end(c, e);
-        c->prof_fields[pk] = kind_fields(kind, a);
+        c->prof_fields[pk] = kind_fields(kind, dir, a);
     }
     if (err != hipSuccess) return fail(c, -4, "pass %s dir %d: %s", kind_names[kind], dir, hipGetErrorString(err));
     return 0;
@@ -529,28 +528,26 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
     const int hyp = c->p.hyper_param;
     st::PassArgs a = base;
     a.first = 1;
-    // the z and x passes write their own partial sums (no read-modify-write);
-    // the y pass adds them up with its own terms and the viscous products
+    // the x and z passes write their own partial sums (no read-modify-write);
+    // the y pass adds them up with its own terms and the viscous products.
+    // z and y also take the mixed second derivatives, from the x (and z)
+    // derivatives stored before them.
     {
         st::PassArgs b = a;
-        b.dQ = c->dQp;
-        b.L2e = hyp ? c->L2ep : nullptr;
-        TRY(pass(c, st::K_GRAD, 2, b));
         b.dQ = c->dQp + 5 * c->N;
         b.L2e = hyp ? c->L2ep + c->N : nullptr;
         TRY(pass(c, st::K_GRAD, 0, b));
+        b.dQ = c->dQp;
+        b.L2e = hyp ? c->L2ep : nullptr;
+        TRY(pass(c, st::K_GRAD, 2, b));
     }
     TRY(pass(c, st::K_GRAD_CROSS, 1, a));
-    a.first = 0;
     if (hyp) {
-        a.first = 1;
-        TRY(pass(c, st::K_MIXH_X, 0, a));
+        // D2 of |S| and Lap e, summed over x, y, z
+        TRY(pass(c, st::K_LAPSE, 0, a));
         a.first = 0;
-        TRY(pass(c, st::K_MIXH_Y, 1, a));
-        TRY(pass(c, st::K_MIXH_Z, 2, a));
-    } else {
-        TRY(pass(c, st::K_MIX_Y, 1, a));
-        TRY(pass(c, st::K_MIX_Z, 2, a));
+        TRY(pass(c, st::K_LAPSE, 1, a));
+        TRY(pass(c, st::K_LAPSE, 2, a));
     }
     return 0;
 }

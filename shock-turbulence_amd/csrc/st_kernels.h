@@ -50,7 +50,7 @@ __device__ __forceinline__ void slot_params(const PassArgs &a, int slot, PassArg
 enum PassKind {
     K_D1, K_D2, K_FILT, K_GAUSS,
     K_GRAD,
-    K_MIX_Y, K_MIX_Z, K_MIXH_X, K_MIXH_Y, K_MIXH_Z,
+    K_LAPSE,
     K_LAP2, K_LAP2_LAST,
     K_GAUSS_ETA, K_GAUSS_ETA_LAST,
     K_HDIV,

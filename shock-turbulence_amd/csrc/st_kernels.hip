@@ -25,11 +25,7 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     switch (kind) {
     case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
     case K_GRAD_CROSS: c = {1, 256}; break;
-    case K_MIX_Y: c = {1, 256}; break;
-    case K_MIX_Z: c = {1, 512}; break;
-    case K_MIXH_X: c = {0, 512}; break;
-    case K_MIXH_Y: c = {1, 256}; break;
-    case K_MIXH_Z: c = {1, 512}; break;
+    case K_LAPSE: c = dir == 2 ? C{1, 512} : C{0, 512}; break;
     case K_LAP2: c = dir == 0 ? C{0, 512} : C{0, 1024}; break;
     case K_LAP2_LAST: c = {0, 1024}; break;
     case K_GAUSS_ETA: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
@@ -52,7 +48,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
         return launch_ops(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_GRAD: case K_GRAD_CROSS:
         return launch_grad(kind, t, nt, dir, My, Mz, Mx, a, s);
-    case K_MIX_Y: case K_MIX_Z: case K_MIXH_X: case K_MIXH_Y: case K_MIXH_Z:
+    case K_LAPSE:
         return launch_mix(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV:
         return launch_hyper(kind, t, nt, dir, My, Mz, Mx, a, s);

@@ -21,25 +21,37 @@ namespace st {
 // all three differ from the reference's evaluation only by rounding.
 // ---------------------------------------------------------------------
 // MW: occupancy target (waves per SIMD) handed to the register allocator;
-// 3 suits the unit-stride y lines, the strided x and z lines run faster with
-// the compiler's own choice.  LEAN selects the operator order that keeps
+// 2 for the y pass, whose nine staged fields allow two 256-thread
+// workgroups per CU anyway; the strided x and z lines run faster with the
+// compiler's own choice.  LEAN selects the operator order that keeps
 // fewer values live (for 1024-thread tiles, which allow 128 registers).
-// CROSS (y lines, after the z and x passes have left their partial sums):
+// CROSS (y lines, after the x and z passes have left their partial sums):
 // the store phase also assembles the right-hand side of the first-derivative
 // part -- the y terms, the z and x partials and the viscous terms built from
 // products of first derivatives -- and |S| and C_beta, from the derivatives
 // of all three directions at the point (the y ones carried from the early
 // store in registers).
-template <int MW, bool LEAN, bool CROSS = false>
+// NMX: stored first derivatives staged as well, whose D1 along the sweep
+// gives the mixed second derivatives of the viscous terms
+// (src/rhsv_periodic_sf.f:1062-1070,1281-1363):
+//   y (4): D1y of du/dx, dv/dx, dv/dz, dw/dz;  z (2): D1z of du/dx, dw/dx
+// each times (1/3 + bulk) mu into the momentum component it belongs to and,
+// times that velocity, into the energy.  The x sweep runs first, so the z
+// and y sweeps find the derivatives they need.
+template <int MW, bool LEAN, bool CROSS, int NMX>
 struct PassGrad {
-    static constexpr int NSLOT = 5, NOUT = 5, NEARLY = 5;
+    static constexpr int NSLOT = 5 + NMX, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
     static constexpr int NCARRY = CROSS ? 4 : 0;
     static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
-    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[5])
+    // G index of mixed input m: y: gxx=0, gyx=1, gyz=7, gzz=8; z: gxx=0, gzx=2
+    DEV static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
+    // momentum component that D1 of mixed input m feeds
+    DEV static constexpr int mdst(int m) { return NMX == 4 ? (m == 1 ? 0 : m == 2 ? 2 : 1) : (m == 0 ? 2 : 0); }
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
         v[0] = pr.u;
@@ -47,6 +59,29 @@ struct PassGrad {
         v[2] = pr.w;
         v[3] = pr.T;
         v[4] = a.Q[3 * a.N + gi];
+#pragma unroll
+        for (int m = 0; m < NMX; m++) v[5 + m] = a.G[msrc(m) * a.N + gi];
+    }
+    // The mixed-derivative terms, first (the velocities are still staged).
+    template <int M, int P, bool NP>
+    DEV static void compute_mix(const PassArgs &a, const Lane<M, P, NP> &L, const double (&mu)[M], double (&out)[5][M])
+    {
+        const double cb = a.ph.bulk + 1.0 / 3.0;
+#pragma unroll 1
+        for (int m = 0; m < NMX; m++) {
+            const int comp = mdst(m);
+            double f[M], w[M + 2 * HALO], r[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) f[k] = L.ld(5 + m, k);
+            window<M>(f, L, w);
+            op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const double t = (cb * mu[k]) * r[k];
+                acc<0x07, M>(out, comp, k, t);
+                out[4][k] += L.ld(comp, k) * t;
+            }
+        }
     }
     // Accumulate into row i of out (one of the rows in MASK) with static
     // register indices.
@@ -76,6 +111,7 @@ struct PassGrad {
 #pragma unroll
             for (int i = 0; i < 5; i++) out[i][k] = 0.0;
         }
+        if constexpr (NMX > 0) compute_mix<M, P, NP>(a, L, mu, out);
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
         for (int it = 0; it < 4; it++) {
@@ -168,6 +204,7 @@ struct PassGrad {
 #pragma unroll
             for (int i = 0; i < 5; i++) out[i][k] = 0.0;
         }
+        if constexpr (NMX > 0) compute_mix<M, P, NP>(a, L, mu, out);
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
         for (int it = 0; it < 4; it++) {
@@ -272,7 +309,7 @@ struct PassGrad {
     }
 
     // CROSS: G(., y) and dT/dy go out and stay in the carry; the Laplacian
-    // of e adds the z and x partials in the order the passes ran.
+    // of e adds the z, then the x partial.
     DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v, double *cr)
     {
         const long N = a.N;
@@ -332,17 +369,21 @@ struct PassGrad {
 
 hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
-    switch (kind) {
-    case K_GRAD:
-        if (nt == 1024) return launch<PassGrad<1, true>>(t, nt, dir, My, Mz, Mx, a, s);
-        if (dir == 1) return launch<PassGrad<3, false>, false>(t, nt, dir, My, Mz, Mx, a, s);
-        return launch<PassGrad<1, false>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_GRAD_CROSS:
-        if (dir != 1) return hipErrorInvalidValue;
-        if (nt == 1024) return launch<PassGrad<1, true, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
-        return launch<PassGrad<3, false, true>, false>(t, nt, dir, My, Mz, Mx, a, s);
-    default: return hipErrorInvalidValue;
+    // x first (no mixed terms), z with two, y last with four and the
+    // pointwise assembly
+    if (kind == K_GRAD && dir == 0) {
+        if (nt == 1024) return launch<PassGrad<1, true, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1, false, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
     }
+    if (kind == K_GRAD && dir == 2) {
+        if (nt == 1024) return launch<PassGrad<1, true, false, 2>, false>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1, false, false, 2>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    }
+    if (kind == K_GRAD_CROSS && dir == 1) {
+        if (nt == 1024) return launch<PassGrad<1, true, true, 4>, false>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<2, false, true, 4>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace st
