@@ -581,11 +581,15 @@ int seq_stage(st_ctx *c, const st::PassArgs &base, double ra, double rb, double 
     a.rk_b = rb;
     a.dt = dt;
     TRY(seq_rhs(c, base));
-    // the update joins the last divergence pass unless the sponge comes
-    // between them
-    const bool fuse = c->p.hyper_param && !c->spg && !getenv("ST_NO_FUSE_RK");
+    // the update (and the sponge term before it) joins the last divergence
+    // pass
+    const bool fuse = c->p.hyper_param && !getenv("ST_NO_FUSE_RK");
+    if (fuse && c->spg) {
+        a.spg = c->spg;
+        a.spg_left = c->spg_left;
+    }
     TRY(seq_hyper(c, base, fuse ? &a : nullptr));
-    if (c->spg) {                                            // the hook order of src/lddrk4_nonperiodic_mf.f:462-466
+    if (c->spg && !fuse) {                                   // the hook order of src/lddrk4_nonperiodic_mf.f:462-466
         const hipError_t e = st::run_sponge(base, c->spg, c->Mx, c->spg_left, c->s);
         if (e != hipSuccess) return fail(c, -4, "sponge: %s", hipGetErrorString(e));
     }

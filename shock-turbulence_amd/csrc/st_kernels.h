@@ -37,6 +37,11 @@ struct PassArgs {
     const int *act;       // per-slot flag (device): 0 skips the slot; null: all
     const double *sre;    // per-slot 1/Re (device); null: ph.re_inv
     const double *sdt;    // per-slot dt (device); null: dt
+    // sponge folded into the stage update: sigma per x index, then the left
+    // and right reference states (5 each); null: no sponge in the update
+    const double *spg;
+    int spg_left;         // x indices below this take the left state
+    int pad3_;
 };
 
 // The parameters of slot `slot` (its 1/Re and dt) over the launch-wide ones.

@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_sponge(PassArgs a, const double *spg, i
         const double *ref = spg + Mx + (ix < nleft ? 0 : 5);
         const long gi = off + i;
 #pragma unroll
-        for (int l = 0; l < 5; l++) a.dQ[l * N + gi] = a.dQ[l * N + gi] - sg * (a.Q[l * N + gi] - ref[l]);
+        for (int l = 0; l < 5; l++) a.dQ[l * N + gi] = fma(-sg, a.Q[l * N + gi] - ref[l], a.dQ[l * N + gi]);
     }
 }
 

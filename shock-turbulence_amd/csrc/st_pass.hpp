@@ -26,8 +26,9 @@ struct carry_n<T, std::void_t<decltype(T::NCARRY)>> { static constexpr int value
 
 // MID of a pass (false when it names none): a second compute phase.  After
 // compute(), mid() rewrites the NSLOT staged values of each point in memory
-// order (it may read and write global fields of its own points), then
-// compute2() runs on the rewritten slots in the line mapping.
+// order (it may read and write global fields of its own points; it also gets
+// the point's index along the line), then compute2() runs on the rewritten
+// slots in the line mapping.
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
 template <class T>
@@ -103,7 +104,7 @@ k_pass(Geo g, PassArgs a0)
             double v[NS];
 #pragma unroll
             for (int q = 0; q < NS; q++) v[q] = lds[q * ss + li];
-            PASS::mid(g, a, gi, v);
+            PASS::mid(g, a, gi, s, v);
 #pragma unroll
             for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
         }

@@ -9,9 +9,10 @@ namespace st {
 // the first of the filter, so one x sweep ends the right-hand side and
 // starts the stage update (src/lddrk4_periodic_sf.f:344-356):
 //   compute   D1x of the staged fluxes (as the hyper_div pass)
-//   mid       per point: dQ = dQ + D1x terms (continuity unchanged),
-//             QW = a QW + dt dQ, Q* = Q + b QW (src/lddrk4_periodic_sf.f:
-//             349-354), Q* staged in the five slots
+//   mid       per point: dQ = dQ + D1x terms (continuity unchanged), the
+//             sponge term when one is set, QW = a QW + dt dQ,
+//             Q* = Q + b QW (src/lddrk4_periodic_sf.f:349-354), Q* staged
+//             in the five slots
 //   compute2  x filter of Q* (Compact_Filter_X, src/filters_periodic.f:115)
 //   store     Q = filtered Q*
 // The right-hand side itself is not written back: nothing reads it after
@@ -38,7 +39,7 @@ struct PassHdivRkf {
     {
         PassHyperDiv::compute<M, P, NP>(g, a, L, o);
     }
-    DEV static void mid(const Geo &, const PassArgs &a, long gi, double (&v)[5])
+    DEV static void mid(const Geo &g, const PassArgs &a, long gi, int ix, double (&v)[5])
     {
         const long N = a.N;
         double dq[5];
@@ -47,12 +48,18 @@ struct PassHdivRkf {
         dq[2] = a.dQ[2 * N + gi] + v[2];
         dq[3] = a.dQ[3 * N + gi];
         dq[4] = a.dQ[4 * N + gi] + v[3];
+        // Sponge_Forcing after Hyper_RHS (src/lddrk4_nonperiodic_mf.f:462-466),
+        // as k_sponge rounds it; ix is the x index (the sweep is along x)
+        const double sg = a.spg ? a.spg[ix] : 0.0;
+        const double *ref = a.spg ? a.spg + g.Mx + (ix < a.spg_left ? 0 : 5) : nullptr;
 #pragma unroll
         for (int c = 0; c < 5; c++) {
             const long o = c * N + gi;
+            const double q = a.Q[o];
+            if (sg != 0.0) dq[c] = fma(-sg, q - ref[c], dq[c]);
             const double qw = fma(a.dt, dq[c], a.rk_a * a.QW[o]);   // as PassFilterQ<0>
             a.QW[o] = qw;
-            v[c] = fma(a.rk_b, qw, a.Q[o]);
+            v[c] = fma(a.rk_b, qw, q);
         }
     }
     template <int M, int P, bool NP>

@@ -106,7 +106,7 @@ struct PassGaussHdiv {
             for (int k = 0; k < M; k++) L.st(m, k, r[k]);
         }
     }
-    DEV static void mid(const Geo &g, const PassArgs &a, long gi, double (&v)[4])
+    DEV static void mid(const Geo &g, const PassArgs &a, long gi, int, double (&v)[4])
     {
         const Phys &ph = a.ph;
         const long N = a.N;

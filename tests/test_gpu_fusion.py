@@ -1,7 +1,7 @@
 """The fused stage update: the last (x) pass of the artificial-transport
-divergence also applies the low-storage RK update and the x filter.  The
-fused and the separate passes (ST_NO_FUSE_RK=1) round alike, so a march
-gives the same bits either way; both are checked against the oracle by the
+divergence also applies the sponge term (when set), the low-storage RK
+update and the x filter.  The fused and the separate passes
+(ST_NO_FUSE_RK=1) round alike, so a march gives the same bits either way; both are checked against the oracle by the
 parity tests."""
 import os
 import sys
@@ -19,13 +19,21 @@ def _log(*a):
     sys.stdout.flush()
 
 
-def _march(shape, nsteps, fuse, **kw):
-    Q = st_amd.ic_isotropic(*shape, k0=4.0, urms=1.0, mt=0.3, seed=77)
+PER = (2.0, 2 * np.pi, 2 * np.pi)
+
+
+def _march(shape, nsteps, fuse, sponge=False, **kw):
+    if sponge:
+        Q, s1, s2 = st_amd.ic_shock(*shape, PER, mach=1.5, frac=0.25, urms=0.1, seed=3)
+    else:
+        Q = st_amd.ic_isotropic(*shape, k0=4.0, urms=1.0, mt=0.3, seed=77)
     if not fuse:
         os.environ["ST_NO_FUSE_RK"] = "1"
     try:
         s = st_amd.Solver(*shape, **kw)
         try:
+            if sponge:
+                s.sponge(5.0, 2.0, 0.4, s1, s2)
             s.upload(Q)
             t, st, dts = s.march(nsteps)
             return s.download(), t, dts
@@ -36,8 +44,10 @@ def _march(shape, nsteps, fuse, **kw):
 
 
 @pytest.mark.parametrize("shape,kw", [((32, 32, 32), {}), ((64, 16, 32), {}),
-                                       ((64, 16, 16), {"nonperiodic_x": 1, "period": (2.0, 2 * np.pi, 2 * np.pi)})])
+                                       ((64, 16, 16), {"nonperiodic_x": 1, "period": PER}),
+                                       ((64, 16, 16), {"nonperiodic_x": 1, "period": PER, "sponge": True})])
 def test_fused_update_same_bits(shape, kw):
+    """With the sponge set, its term is folded into the fused update too."""
     _log("fused vs separate update", shape, kw)
     a = _march(shape, 3, True, Re=150.0, hyper_param=1, **kw)
     b = _march(shape, 3, False, Re=150.0, hyper_param=1, **kw)
