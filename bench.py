@@ -33,8 +33,8 @@ STAGES_PER_STEP = 11
 # kernel template of each pass kind (csrc/st_pass_*.hip)
 PASS_STRUCT = {
     "lap2": "PassLap2<false>", "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta",
-    "gauss_hdiv": "PassGaussHdiv", "hyper_div": "PassHyperDiv", "rk_filter_x": "PassFilterQ<0>",
-    "filter_y": "PassFilterQ<1>", "filter_z_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
+    "gauss_hdiv": "PassGaussHdiv", "hyper_div": "PassHyperDiv", "rk_filter": "PassFilterQ<0>",
+    "filter_q": "PassFilterQ<1>", "filter_q_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
 }
 
 

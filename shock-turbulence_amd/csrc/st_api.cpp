@@ -23,8 +23,8 @@ static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds
 // twice); the algorithmic bytes of a launch are this times 8 B per point.
 const char *kind_names[NKIND] = {
     "d1", "d2", "filter", "gauss", "grad", "lap_se",
-    "lap2", "lap2_eta", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter_x", "filter_y",
-    "filter_z_fix", "grad_cross", "hdiv_rkf", "", "", "", "", "", "reduce", "fill", ""};
+    "lap2", "lap2_eta", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter", "filter_q",
+    "filter_q_fix", "grad_cross", "hdiv_rkf", "", "", "", "", "", "reduce", "fill", ""};
 
 struct Event {
     hipEvent_t a, b;
@@ -176,9 +176,9 @@ double kind_fields(int kind, int dir, const st::PassArgs &a)
     // eta, C_beta, beta* mu* kappa* (written), the z-flux inputs (8 G, 4 Q), dQ 4 (read and written)
     case st::K_GAUSS_ETA_LAST: return 2 + (a.Cb ? 1 : 0) + 3 + 8 + 4 + 8;
     case st::K_HDIV: return 3 + 8 + 4 + 8;
-    case st::K_RKF_X: return 15 + 5 + 5;
-    case st::K_F_Y: return 10;
-    case st::K_F_Z: return 10;
+    case st::K_RKF: return 15 + 5 + 5;
+    case st::K_FQ: return 10;
+    case st::K_FQ_FIX: return 10;
     // Q, G of y (write), the 4 inputs of the mixed terms, G of x and z (8),
     // the z and x partials, dQ; with the
     // artificial transport the Laplacian of e (2 partials + the sum), |S|
@@ -552,9 +552,10 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
     return 0;
 }
 
-// Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ.  With rk set, the last
-// (x) divergence pass also applies the stage update and the x filter
-// (K_HDIV_RKF), so the stage goes on with the y and z filters.
+// Hyper_RHS (src/hyper_rhs.f:12-499) added into dQ, directions z, x, y.
+// With rk set, the last (y) divergence pass also applies the stage update
+// and the y filter (K_HDIV_RKF), so the stage goes on with the z and x
+// filters.
 int seq_hyper(st_ctx *c, const st::PassArgs &base, const st::PassArgs *rk = nullptr)
 {
     if (!c->p.hyper_param) return 0;
@@ -567,9 +568,9 @@ int seq_hyper(st_ctx *c, const st::PassArgs &base, const st::PassArgs *rk = null
     TRY(pass(c, st::K_GAUSS_ETA, 0, a));
     TRY(pass(c, st::K_GAUSS_ETA, 1, a));
     TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));            // and the z divergence
-    TRY(pass(c, st::K_HDIV, 1, a));
-    if (rk) return pass(c, st::K_HDIV_RKF, 0, *rk);
     TRY(pass(c, st::K_HDIV, 0, a));
+    if (rk) return pass(c, st::K_HDIV_RKF, 1, *rk);
+    TRY(pass(c, st::K_HDIV, 1, a));
     return 0;
 }
 
@@ -593,10 +594,11 @@ int seq_stage(st_ctx *c, const st::PassArgs &base, double ra, double rb, double 
         const hipError_t e = st::run_sponge(base, c->spg, c->Mx, c->spg_left, c->s);
         if (e != hipSuccess) return fail(c, -4, "sponge: %s", hipGetErrorString(e));
     }
-    if (!fuse) TRY(pass(c, st::K_RKF_X, 0, a));
-    TRY(pass(c, st::K_F_Y, 1, a));
+    // filters y (with the update), z, x (with Fix_Pressure)
+    if (!fuse) TRY(pass(c, st::K_RKF, 1, a));
+    TRY(pass(c, st::K_FQ, 2, a));
     a.fix = 1;
-    TRY(pass(c, st::K_F_Z, 2, a));
+    TRY(pass(c, st::K_FQ_FIX, 0, a));
     return 0;
 }
 

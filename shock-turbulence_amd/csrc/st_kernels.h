@@ -59,7 +59,7 @@ enum PassKind {
     K_LAP2, K_LAP2_LAST,
     K_GAUSS_ETA, K_GAUSS_ETA_LAST,
     K_HDIV,
-    K_RKF_X, K_F_Y, K_F_Z,
+    K_RKF, K_FQ, K_FQ_FIX,
     K_GRAD_CROSS,
     K_HDIV_RKF,
     K_NPASS,              // pass kinds end here

@@ -25,16 +25,16 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     switch (kind) {
     case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
     case K_GRAD_CROSS: c = {1, 256}; break;
-    case K_LAPSE: c = dir == 2 ? C{1, 512} : C{0, 512}; break;
-    case K_LAP2: c = dir == 0 ? C{0, 512} : C{0, 1024}; break;
+    case K_LAPSE: c = dir == 1 ? C{1, 256} : C{0, 512}; break;
+    case K_LAP2: c = dir == 0 ? C{0, 1024} : C{1, 256}; break;
     case K_LAP2_LAST: c = {0, 1024}; break;
-    case K_GAUSS_ETA: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
+    case K_GAUSS_ETA: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
     case K_GAUSS_ETA_LAST: c = {0, 512}; break;
     case K_HDIV: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
-    case K_RKF_X: c = {0, 512}; break;
-    case K_HDIV_RKF: c = {0, 512}; break;
-    case K_F_Y: c = {0, 256}; break;
-    case K_F_Z: c = {0, 256}; break;
+    case K_HDIV_RKF: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_RKF: c = {0, 256}; break;
+    case K_FQ: c = {0, 256}; break;
+    case K_FQ_FIX: c = {0, 256}; break;
     default: break;
     }
     *narrow = c.n;
@@ -52,7 +52,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
         return launch_mix(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV:
         return launch_hyper(kind, t, nt, dir, My, Mz, Mx, a, s);
-    case K_RKF_X: case K_F_Y: case K_F_Z:
+    case K_RKF: case K_FQ: case K_FQ_FIX:
         return launch_filter(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_HDIV_RKF:
         return launch_hdivrk(kind, t, nt, dir, My, Mz, Mx, a, s);

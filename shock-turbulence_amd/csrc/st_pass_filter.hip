@@ -4,11 +4,13 @@
 namespace st {
 
 // ---------------------------------------------------------------------
-// Low-storage RK update fused into the x pass of the compact filter
-// (src/lddrk4_periodic_sf.f:349-354), then the y and z filter passes; the
-// z pass applies Fix_Pressure (src/lddrk4_periodic_sf.f:436-473).
+// Low-storage RK update fused into the first pass of the compact filter
+// (src/lddrk4_periodic_sf.f:349-354), then the other two filter passes; the
+// last applies Fix_Pressure (src/lddrk4_periodic_sf.f:436-473).  The stage
+// filters y, z, x: the three line filters commute, so the order changes
+// only the rounding.
 // ---------------------------------------------------------------------
-template <int KIND>   // 0: RK update + x filter, 1: y filter, 2: z filter + pressure fix
+template <int KIND>   // 0: RK update + filter, 1: filter, 2: filter + pressure fix
 struct PassFilterQ {
     static constexpr int NSLOT = 5, NOUT = 0, NEARLY = 5;
     static constexpr bool REUSE = true;
@@ -74,9 +76,9 @@ struct PassFilterQ {
 hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     switch (kind) {
-    case K_RKF_X: return launch<PassFilterQ<0>>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_F_Y: return launch<PassFilterQ<1>, false>(t, nt, dir, My, Mz, Mx, a, s);
-    case K_F_Z: return launch<PassFilterQ<2>, false>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_RKF: return launch<PassFilterQ<0>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_FQ: return launch<PassFilterQ<1>>(t, nt, dir, My, Mz, Mx, a, s);
+    case K_FQ_FIX: return launch<PassFilterQ<2>>(t, nt, dir, My, Mz, Mx, a, s);
     default: return hipErrorInvalidValue;
     }
 }

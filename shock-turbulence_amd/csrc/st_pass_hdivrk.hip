@@ -1,19 +1,21 @@
-// The x pass of the artificial-transport divergence fused with the
-// low-storage RK update and the x pass of the compact filter.
+// The last pass of the artificial-transport divergence fused with the
+// low-storage RK update and the first pass of the compact filter.
 #include "st_pass_hdiv.hpp"
 
 namespace st {
 
 // ---------------------------------------------------------------------
-// x is the last direction of the divergence (src/hyper_rhs.f:159-491) and
-// the first of the filter, so one x sweep ends the right-hand side and
-// starts the stage update (src/lddrk4_periodic_sf.f:344-356):
-//   compute   D1x of the staged fluxes (as the hyper_div pass)
-//   mid       per point: dQ = dQ + D1x terms (continuity unchanged), the
+// The last direction of the divergence (src/hyper_rhs.f:159-491) is the
+// first of the filter (the stage runs the divergence z, x, y and the filter
+// y, z, x), so one sweep ends the right-hand side and starts the stage
+// update (src/lddrk4_periodic_sf.f:344-356):
+//   compute   D1 of the staged fluxes (as the hyper_div pass)
+//   mid       per point: dQ = dQ + D1 terms (continuity unchanged), the
 //             sponge term when one is set, QW = a QW + dt dQ,
 //             Q* = Q + b QW (src/lddrk4_periodic_sf.f:349-354), Q* staged
 //             in the five slots
-//   compute2  x filter of Q* (Compact_Filter_X, src/filters_periodic.f:115)
+//   compute2  filter of Q* along the sweep (Compact_Filter_X/Y/Z,
+//             src/filters_periodic.f:115,254,393)
 //   store     Q = filtered Q*
 // The right-hand side itself is not written back: nothing reads it after
 // the stage update.  Rounding is that of the two passes run apart.
@@ -49,7 +51,8 @@ struct PassHdivRkf {
         dq[3] = a.dQ[3 * N + gi];
         dq[4] = a.dQ[4 * N + gi] + v[3];
         // Sponge_Forcing after Hyper_RHS (src/lddrk4_nonperiodic_mf.f:462-466),
-        // as k_sponge rounds it; ix is the x index (the sweep is along x)
+        // as k_sponge rounds it; sigma and the reference state by x index
+        if (a.spg && g.dir != 0) ix = (int)((gi % a.ni) / ((long)g.My * g.Mz));
         const double sg = a.spg ? a.spg[ix] : 0.0;
         const double *ref = a.spg ? a.spg + g.Mx + (ix < a.spg_left ? 0 : 5) : nullptr;
 #pragma unroll
@@ -89,7 +92,7 @@ struct PassHdivRkf {
 hipError_t launch_hdivrk(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a,
                          hipStream_t s)
 {
-    if (kind != K_HDIV_RKF || dir != 0) return hipErrorInvalidValue;
+    if (kind != K_HDIV_RKF || dir == 2) return hipErrorInvalidValue;
     return launch<PassHdivRkf>(t, nt, dir, My, Mz, Mx, a, s);
 }
 

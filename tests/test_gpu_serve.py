@@ -61,7 +61,7 @@ def test_autotune_keeps_results():
     try:
         b.load(0, Q, 200.0, 3)
         cfg = b.autotune(1)
-        _log("tuned", {k: v for k, v in cfg.items() if k.split(".")[0] in ("grad", "hyper_div", "rk_filter_x")})
+        _log("tuned", {k: v for k, v in cfg.items() if k.split(".")[0] in ("grad", "hyper_div", "hdiv_rkf")})
         assert np.array_equal(b.store(0), Q)
         b.march(3)
         b.sync()
