@@ -71,6 +71,29 @@ def traffic_of(kernel):
         return None
 
 
+def copy_bandwidth(nbytes=1 << 31, reps=5):
+    """Measured device copy bandwidth (GB/s of bytes read + written): the
+    faster of a device-to-device copy and an elementwise scaled copy of
+    nbytes, timed with events over reps launches."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    best = 0.0
+    for op in (lambda: y.copy_(x), lambda: torch.mul(x, 1.0, out=y)):
+        op()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op()
+        e1.record()
+        torch.cuda.synchronize()
+        best = max(best, 2.0 * nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9)
+    del x, y
+    torch.cuda.empty_cache()
+    return round(best, 1)
+
+
 def cpu_baseline(n, threads, Q, kw):
     """Time the CPU oracle (C restatement, OpenMP) on one RK stage of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -196,6 +219,7 @@ def main():
     ap.add_argument("--serve-runs", type=int, default=16, help="serving leg: runs in the trace (0: skip)")
     ap.add_argument("--config2-steps", type=int, default=3, help="non-periodic 512x192x192 leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--copy-bw", type=int, default=1, help="also measure a 2 GiB device copy (GB/s), before the timed region")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 2 GiB fill and a 2 GiB copy first (reference kernels for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -207,6 +231,9 @@ def main():
     import numpy as np
     import st_amd
 
+    # torch's HIP runtime must initialise the device before this library's
+    # does, so the copy is measured first
+    copy_gbs = copy_bandwidth() if args.copy_bw else None
     if args.calib_copy:
         # reference kernels of known traffic for the byte counters: a 2 GiB
         # fill (2 GiB written) and a 2 GiB scaled copy (2 GiB read + written)
@@ -288,6 +315,9 @@ def main():
                 "alg_bytes_per_launch": alg}
     stage_roof = {"B_alg_per_pt_stage": B_ALG, "achieved_GBs": round(B_ALG * pts * stages / el / 1e9, 1),
                   "frac": round(B_ALG * pts * stages / el / 1e9 / HBM_PEAK_GBS, 4)}
+    if copy_gbs:
+        stage_roof["measured_copy_GBs"] = copy_gbs
+        stage_roof["frac_of_copy"] = round(stage_roof["achieved_GBs"] / copy_gbs, 4)
 
     cpu = None
     if not args.no_cpu:
