@@ -32,7 +32,7 @@ STAGES_PER_STEP = 11
 
 # kernel template of each pass kind (csrc/st_pass_*.hip)
 PASS_STRUCT = {
-    "lap2": "PassLap2<false>", "lap2_eta": "PassLap2<true>", "gauss_eta": "PassGaussEta",
+    "lap2": "PassLap2", "lap2_gauss": "PassLap2Gauss", "lap_se_lap2": "PassLapSELap2", "gauss_eta": "PassGaussEta",
     "gauss_hdiv": "PassGaussHdiv", "hyper_div": "PassHyperDiv", "rk_filter": "PassFilterQ<0>",
     "filter_q": "PassFilterQ<1>", "filter_q_fix": "PassFilterQ<2>", "hdiv_rkf": "PassHdivRkf",
 }

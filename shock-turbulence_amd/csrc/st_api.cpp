@@ -23,8 +23,8 @@ static_assert(st::K_NPASS <= KIND_RED, "pass kinds overlap the other timed kinds
 // twice); the algorithmic bytes of a launch are this times 8 B per point.
 const char *kind_names[NKIND] = {
     "d1", "d2", "filter", "gauss", "grad", "lap_se",
-    "lap2", "lap2_eta", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter", "filter_q",
-    "filter_q_fix", "grad_cross", "hdiv_rkf", "", "", "", "", "", "reduce", "fill", ""};
+    "lap2", "lap2_gauss", "gauss_eta", "gauss_hdiv", "hyper_div", "rk_filter", "filter_q",
+    "filter_q_fix", "grad_cross", "hdiv_rkf", "lap_se_lap2", "", "", "", "", "reduce", "fill", ""};
 
 struct Event {
     hipEvent_t a, b;
@@ -171,7 +171,10 @@ double kind_fields(int kind, int dir, const st::PassArgs &a)
     case st::K_GRAD: return 5 + 4 + 5 + (a.L2e ? 1 : 0) + (dir == 2 ? 2 : 0);
     case st::K_LAPSE: return a.first ? 2 + 2 : 2 + 4;
     case st::K_LAP2: return a.first ? 2 : 3;
+    // L2S, L4S, L4e, Q in; eta out
     case st::K_LAP2_LAST: return 1 + 1 + 1 + 5 + 2;
+    // S, Lap e and the L2S, L4e partials in; L2S, L4e, L4S out
+    case st::K_LAPSE_LAP2: return 4 + 3;
     case st::K_GAUSS_ETA: return 4;
     // eta, C_beta, beta* mu* kappa* (written), the z-flux inputs (8 G, 4 Q), dQ 4 (read and written)
     case st::K_GAUSS_ETA_LAST: return 2 + (a.Cb ? 1 : 0) + 3 + 8 + 4 + 8;
@@ -522,7 +525,8 @@ int fill(st_ctx *c, double *p, int nfields, const st::PassArgs &base, double v)
     } while (0)
 
 // RHS (src/rhsv_periodic_sf.f:9-25) into dQ; with the artificial transport
-// on, also the strain magnitude, sensor and first Laplacians it needs.
+// on, also the strain magnitude, the sensor and the x and y parts of the
+// first Laplacians it needs.
 int seq_rhs(st_ctx *c, const st::PassArgs &base)
 {
     const int hyp = c->p.hyper_param;
@@ -543,11 +547,11 @@ int seq_rhs(st_ctx *c, const st::PassArgs &base)
     }
     TRY(pass(c, st::K_GRAD_CROSS, 1, a));
     if (hyp) {
-        // D2 of |S| and Lap e, summed over x, y, z
+        // D2 of |S| and Lap e, summed over x, y (z joins the first pass of
+        // the artificial transport)
         TRY(pass(c, st::K_LAPSE, 0, a));
         a.first = 0;
         TRY(pass(c, st::K_LAPSE, 1, a));
-        TRY(pass(c, st::K_LAPSE, 2, a));
     }
     return 0;
 }
@@ -561,12 +565,13 @@ int seq_hyper(st_ctx *c, const st::PassArgs &base, const st::PassArgs *rk = null
     if (!c->p.hyper_param) return 0;
     st::PassArgs a = base;
     a.first = 1;
-    TRY(pass(c, st::K_LAP2, 0, a));
+    // z: the last D2 of |S| and Lap e, then the first of Lap(Lap|S|)
+    TRY(pass(c, st::K_LAPSE_LAP2, 2, a));
     a.first = 0;
-    TRY(pass(c, st::K_LAP2, 1, a));
-    TRY(pass(c, st::K_LAP2_LAST, 2, a));
+    TRY(pass(c, st::K_LAP2, 0, a));
+    // y: the last of Lap(Lap|S|), eta, and the first Gaussian direction
+    TRY(pass(c, st::K_LAP2_LAST, 1, a));
     TRY(pass(c, st::K_GAUSS_ETA, 0, a));
-    TRY(pass(c, st::K_GAUSS_ETA, 1, a));
     TRY(pass(c, st::K_GAUSS_ETA_LAST, 2, a));            // and the z divergence
     TRY(pass(c, st::K_HDIV, 0, a));
     if (rk) return pass(c, st::K_HDIV_RKF, 1, *rk);

@@ -62,6 +62,7 @@ enum PassKind {
     K_RKF, K_FQ, K_FQ_FIX,
     K_GRAD_CROSS,
     K_HDIV_RKF,
+    K_LAPSE_LAP2,
     K_NPASS,              // pass kinds end here
 };
 

@@ -26,8 +26,9 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
     case K_GRAD_CROSS: c = {1, 256}; break;
     case K_LAPSE: c = dir == 1 ? C{1, 256} : C{0, 512}; break;
-    case K_LAP2: c = dir == 0 ? C{0, 1024} : C{1, 256}; break;
-    case K_LAP2_LAST: c = {0, 1024}; break;
+    case K_LAP2: c = {0, 1024}; break;
+    case K_LAP2_LAST: c = {1, 256}; break;
+    case K_LAPSE_LAP2: c = {0, 512}; break;
     case K_GAUSS_ETA: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
     case K_GAUSS_ETA_LAST: c = {0, 512}; break;
     case K_HDIV: c = dir == 0 ? C{0, 512} : C{1, 512}; break;
@@ -50,7 +51,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
         return launch_grad(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_LAPSE:
         return launch_mix(kind, t, nt, dir, My, Mz, Mx, a, s);
-    case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV:
+    case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV: case K_LAPSE_LAP2:
         return launch_hyper(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_RKF: case K_FQ: case K_FQ_FIX:
         return launch_filter(kind, t, nt, dir, My, Mz, Mx, a, s);
