@@ -29,6 +29,14 @@ struct carry_n<T, std::void_t<decltype(T::NCARRY)>> { static constexpr int value
 // order (it may read and write global fields of its own points; it also gets
 // the point's index along the line), then compute2() runs on the rewritten
 // slots in the line mapping.
+// NKEEP of a pass (0 when it names none): the last NKEEP staged input
+// slots, which compute() only reads, are handed to store() after the late
+// results, so the store phase need not read those fields again.
+template <class T, class = void>
+struct keep_n { static constexpr int value = 0; };
+template <class T>
+struct keep_n<T, std::void_t<decltype(T::NKEEP)>> { static constexpr int value = T::NKEEP; };
+
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
 template <class T>
@@ -144,7 +152,9 @@ k_pass(Geo g, PassArgs a0)
         const int k = s % M, c = s / M;
         const int li = lds_idx<M>(b, k, c, P);
         constexpr int NV = RE ? NO : NO + NE;
-        double v[NV + NPF > 0 ? NV + NPF : 1];
+        constexpr int NK = keep_n<PASS>::value;
+        static_assert(NK == 0 || (RE && NS - NK >= NO && NS - NK >= NE), "kept slots hold no results");
+        double v[NV + NK + NPF > 0 ? NV + NK + NPF : 1];
 #pragma unroll
         for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
         if (!RE) {
@@ -152,7 +162,9 @@ k_pass(Geo g, PassArgs a0)
             for (int j = NO; j < NV; j++) v[j] = lds[(NS + j - NO) * ss + li];
         }
 #pragma unroll
-        for (int j = 0; j < NPF; j++) v[NV + j] = pf[r][j];
+        for (int j = 0; j < NK; j++) v[NV + j] = lds[(NS - NK + j) * ss + li];
+#pragma unroll
+        for (int j = 0; j < NPF; j++) v[NV + NK + j] = pf[r][j];
         if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
         else PASS::store(g, a, gi, v);
     }

@@ -43,14 +43,17 @@ struct PassGrad {
     static constexpr int NSLOT = 5 + NMX, NOUT = 5, NEARLY = 5;
     static constexpr bool REUSE = true;
     static constexpr int NCARRY = CROSS ? 4 : 0;
+    // CROSS: the staged inputs of the mixed terms (du/dx, dv/dx, dv/dz,
+    // dw/dz) reach the store after the late results
+    static constexpr int NKEEP = CROSS ? NMX : 0;
     static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     // G index of mixed input m: y: gxx=0, gyx=1, gyz=7, gzz=8; z: gxx=0, gzx=2
-    DEV static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
+    HD static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
     // momentum component that D1 of mixed input m feeds
-    DEV static constexpr int mdst(int m) { return NMX == 4 ? (m == 1 ? 0 : m == 2 ? 2 : 1) : (m == 0 ? 2 : 0); }
+    HD static constexpr int mdst(int m) { return NMX == 4 ? (m == 1 ? 0 : m == 2 ? 2 : 1) : (m == 0 ? 2 : 0); }
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
@@ -333,9 +336,11 @@ struct PassGrad {
         const Phys &ph = a.ph;
         const Prim pr = prim(ph, a.Q[gi], a.Q[N + gi], a.Q[2 * N + gi], a.Q[3 * N + gi], a.Q[4 * N + gi]);
         const double *G = a.G;
-        const double gxx = G[gi], gyx = G[N + gi], gzx = G[2 * N + gi];
+        static_assert(!CROSS || (NMX == 4 && msrc(0) == 0 && msrc(1) == 1 && msrc(2) == 7 && msrc(3) == 8),
+                      "the kept slots are du/dx, dv/dx, dv/dz, dw/dz");
+        const double gxx = v[5], gyx = v[6], gzx = G[2 * N + gi];
         const double gxy = cr[0], gyy = cr[1], gzy = cr[2];
-        const double gxz = G[6 * N + gi], gyz = G[7 * N + gi], gzz = G[8 * N + gi];
+        const double gxz = G[6 * N + gi], gyz = v[7], gzz = v[8];
         const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
         const double tx = mup * G[9 * N + gi], ty = mup * cr[3], tz = mup * G[11 * N + gi];
         const double f1 = ph.bulk - 2.0 / 3.0;
