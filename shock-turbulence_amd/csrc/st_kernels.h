@@ -73,7 +73,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
 // per translation unit (st_pass_*.hip)
 hipError_t launch_ops(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
-hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
+hipError_t launch_lapse(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_hdivrk(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);
 hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s);

@@ -50,7 +50,7 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
     case K_GRAD: case K_GRAD_CROSS:
         return launch_grad(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_LAPSE:
-        return launch_mix(kind, t, nt, dir, My, Mz, Mx, a, s);
+        return launch_lapse(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV: case K_LAPSE_LAP2:
         return launch_hyper(kind, t, nt, dir, My, Mz, Mx, a, s);
     case K_RKF: case K_FQ: case K_FQ_FIX:

@@ -58,7 +58,7 @@ struct PassLapSE {
 };
 
 
-hipError_t launch_mix(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+hipError_t launch_lapse(int kind, const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
 {
     if (kind != K_LAPSE) return hipErrorInvalidValue;
     if (a.first) return launch<PassLapSE<false>>(t, nt, dir, My, Mz, Mx, a, s);
