@@ -123,6 +123,7 @@ def ensemble_leg(nsteps, autotune):
         r.make_ic()
     work = sum(r.n ** 3 * STAGES_PER_STEP * nsteps for r in runs)
     ens = st_serve.Ensemble(runs, autotune=autotune)
+    tuned = dict(ens.tuned)
     ens.march(0)
     t0 = time.perf_counter()
     ens.march(nsteps)
@@ -131,6 +132,8 @@ def ensemble_leg(nsteps, autotune):
     plain = 0.0
     for r in runs:
         s = st_amd.Solver(r.n, r.n, r.n, Re=r.Re, Pr=0.7, hyper_param=1, cfl_cal=0.5)
+        if r.n in tuned:
+            s.set_pass_configs(tuned[r.n])    # the batch's launch shapes: same rounding, same bits
         s.upload(r.Q)
         s.sync()
         t0 = time.perf_counter()
@@ -185,25 +188,25 @@ def config2_leg(nsteps, warmup, autotune=3):
             "finite": finite, "dt_last": float(dts[-1])}
 
 
-def serving_leg(nruns, rate, seed):
+def serving_leg(nruns, rate, seed, autotune=0):
     """A finite Poisson trace of runs (sizes 48^3..128^3, 50..400 steps)
     served with continuous batching, against the plain path one at a time
     (first come, first served on the same GPU)."""
     import st_serve
     runs = st_serve.poisson_trace(nruns, rate=rate, seed=seed)
-    srv = st_serve.Server(runs, budget_bytes=200e9)
+    srv = st_serve.Server(runs, budget_bytes=200e9, autotune=autotune)
     try:
         served = srv.serve()
     finally:
         srv.close()
-    plain, ref = st_serve.plain_one_at_a_time(runs)
+    plain, ref = st_serve.plain_one_at_a_time(runs, configs=srv.tuned)
     same = all(bool((r.result == ref[r.rid]).all()) for r in runs)
     return {"workload": "%d runs, Poisson arrivals at %.0f/s, N in {48,64,96,128}^3 uniform, steps U[50,400], "
                         "seed %d; HBM budget 200 GB" % (nruns, rate, seed),
             "served": {k: round(v, 4) if isinstance(v, float) else v for k, v in served.items()},
             "plain_one_at_a_time": {k: round(v, 4) if isinstance(v, float) else v for k, v in plain.items()},
             "throughput_ratio": round(served["mpts_stage_per_s"] / plain["mpts_stage_per_s"], 3),
-            "results_equal_alone_and_in_mix": same}
+            "results_equal_alone_and_in_mix": same, "autotune_reps": autotune}
 
 
 def main():
@@ -217,7 +220,9 @@ def main():
     ap.add_argument("--profile", type=int, default=1, help="per-pass HIP-event timing (in extra steps after the timed region)")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--ensemble-steps", type=int, default=6, help="batched-ensemble leg (0: skip)")
-    ap.add_argument("--autotune", type=int, default=0, help="autotune the ensemble batches (stages per candidate)")
+    ap.add_argument("--autotune", type=int, default=2,
+                    help="ensemble and serving legs: tune each batch's launch shapes at set-up (stages per "
+                         "candidate; the plain path gets the same shapes); 0: the 256^3 defaults")
     ap.add_argument("--serve-runs", type=int, default=16, help="serving leg: runs in the trace (0: skip)")
     ap.add_argument("--config2-steps", type=int, default=3, help="non-periodic 512x192x192 leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -326,7 +331,7 @@ def main():
         cpu = cpu_baseline(n, args.cpu_threads, Q, kw)
     del Q, Qend
     ensemble = ensemble_leg(args.ensemble_steps, args.autotune) if args.ensemble_steps > 0 else None
-    serving = serving_leg(args.serve_runs, args.serve_rate, 2024) if args.serve_runs > 0 else None
+    serving = serving_leg(args.serve_runs, args.serve_rate, 2024, args.autotune) if args.serve_runs > 0 else None
     config2 = config2_leg(args.config2_steps, 1) if args.config2_steps > 0 else None
 
     out = {

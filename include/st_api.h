@@ -151,9 +151,13 @@ int st_query(st_ctx *ctx);
  * tile width (256, 512, 1024 threads) and keep the fastest per pass and
  * direction; reps stages per candidate.  The state is restored.
  * st_pass_config reads the choice for kind_dir = kind*3 + direction (the
- * index of st_pass_name). */
+ * index of st_pass_name); st_pass_config_set sets it (narrow 0 or 1, nt
+ * 256, 512 or 1024), e.g. to give another context of the same grid the
+ * same lane layouts, and so the same rounding.  A tile width that a pass
+ * cannot use falls back to the next smaller one at launch. */
 int st_autotune(st_ctx *ctx, int reps);
 int st_pass_config(st_ctx *ctx, int kind_dir, int *narrow, int *nt);
+int st_pass_config_set(st_ctx *ctx, int kind_dir, int narrow, int nt);
 
 /* ---- on-disk formats (host only, no context) ----
  * Write_Restart / Read_Restart  src/out.f:493-692: a 28-byte header (the

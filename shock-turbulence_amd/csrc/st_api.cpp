@@ -1152,6 +1152,21 @@ extern "C" int st_pass_config(st_ctx *c, int kind_dir, int *narrow, int *nt)
     return 0;
 }
 
+extern "C" int st_pass_config_set(st_ctx *c, int kind_dir, int narrow, int nt)
+{
+    if (!c || kind_dir < 0 || kind_dir >= NKIND * 3 || (narrow != 0 && narrow != 1) ||
+        (nt != 256 && nt != 512 && nt != 1024))
+        return -1;
+    c->cfg_narrow[kind_dir / 3][kind_dir % 3] = narrow;
+    c->cfg_nt[kind_dir / 3][kind_dir % 3] = nt;
+    for (auto &g : c->graphs)                                  // captured with the old shapes
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return 0;
+}
+
 extern "C" int st_diagnostics(st_ctx *c, double *out7)
 {
     if (!c || !out7) return -1;

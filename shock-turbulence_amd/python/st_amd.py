@@ -60,7 +60,7 @@ EXPORTS = [
     "st_hyper_rhs", "st_get_hyper", "st_set_hyper", "st_get_grad", "st_fix_pressure", "st_cfl", "st_tstep",
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
-    "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_query",
+    "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_pass_config_set", "st_query",
     "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock",
 ]
 
@@ -128,6 +128,7 @@ def lib():
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_ulonglong, _dp, _dp]
     L.st_pass_config.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.st_pass_config_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     _lib = L
     return L
 
@@ -388,6 +389,16 @@ class Solver:
             lib().st_pass_config(self.h, k, ctypes.byref(n), ctypes.byref(t))
             out[name] = (n.value, t.value)
         return out
+
+    def set_pass_configs(self, cfg):
+        """Apply {"kind.direction": (narrow, nt)} (as pass_configs returns)."""
+        idx = {}
+        for k in range(72):
+            name = lib().st_pass_name(k).decode()
+            if name and not name.startswith("."):
+                idx[name] = k
+        for name, (n, t) in cfg.items():
+            self._chk(lib().st_pass_config_set(self.h, idx[name], int(n), int(t)), "st_pass_config_set")
 
     # profiling
     def profile(self, on=True):

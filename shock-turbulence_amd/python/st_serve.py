@@ -13,7 +13,9 @@ streams so their kernels overlap.
   time-to-completion is measured per run.
 
 Both have a one-at-a-time reference on the plain path (one Solver per
-run, st_march), which gives the same bits per run.
+run, st_march), which gives the same bits per run.  When the batches tune
+their launch shapes (st_autotune), the plain path is given the same shapes
+per grid size (st_pass_config_set): the lane layout sets the rounding.
 """
 import collections
 import time
@@ -59,9 +61,20 @@ class Pool:
 
     def __init__(self, n, slots, params):
         self.n = n
+        self.nslots = slots
         self.batch = st_amd.Batch(n, n, n, slots, **params)
         self.free = list(range(slots - 1, -1, -1))
         self.running = {}
+
+    def tune(self, reps, run):
+        """Launch shapes for this size, timed with run's state in every slot
+        (the slots are free again afterwards)."""
+        for slot in range(self.nslots):
+            self.batch.load(slot, run.make_ic(), run.Re, 1)
+        cfg = self.batch.autotune(reps)
+        for slot in range(self.nslots):
+            self.batch.release(slot)
+        return cfg
 
     def admit(self, run, now):
         slot = self.free.pop()
@@ -137,13 +150,17 @@ class Server:
     pools keep their streams busy side by side and a finished run waits at
     most one chunk."""
 
-    def __init__(self, runs, budget_bytes=200e9, slice_s=0.03, **kw):
+    def __init__(self, runs, budget_bytes=200e9, slice_s=0.03, autotune=0, **kw):
         self.runs = sorted(runs, key=lambda r: r.arrival)
         self.slice_s = slice_s
         self.slots = pool_sizes(self.runs, budget_bytes)
         self.pools = {n: Pool(n, s, _params(kw)) for n, s in sorted(self.slots.items())}
         for r in self.runs:
             r.make_ic()                       # a run arrives with its initial state
+        self.tuned = {}
+        if autotune:                          # set-up, before the trace starts
+            for n, p in self.pools.items():
+                self.tuned[n] = p.tune(autotune, next(r for r in self.runs if r.n == n))
 
     def serve(self):
         pending = collections.deque(self.runs)
@@ -209,14 +226,17 @@ def stats(runs, makespan):
     }
 
 
-def plain_one_at_a_time(runs, **kw):
+def plain_one_at_a_time(runs, configs=None, **kw):
     """The plain path: each run alone (st_march) in arrival order, first come
-    first served; returns the same statistics and the per-run results."""
+    first served; returns the same statistics and the per-run results.
+    configs: {n: pass configurations} to apply per grid size."""
     results = {}
     service = {}
     for r in sorted(runs, key=lambda r: r.arrival):
         r.make_ic()
         s = st_amd.Solver(r.n, r.n, r.n, Re=r.Re, **_params(kw))   # set-up is not timed
+        if configs and r.n in configs:
+            s.set_pass_configs(configs[r.n])
         try:
             t1 = time.perf_counter()
             s.upload(r.Q)

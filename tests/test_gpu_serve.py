@@ -75,3 +75,27 @@ def test_autotune_keeps_results():
             s.close()
     finally:
         b.close()
+
+
+def test_tuned_batches_and_plain_path_share_bits():
+    """Batches that tune their launch shapes hand them to the plain path
+    (st_pass_config_set), so every run still gives the same bits alone and
+    in the mix."""
+    runs = [st_serve.Run(i, n, 3, seed=i + 1, k0=(3.0, 4.0, 6.0)[i % 3]) for i, n in enumerate([32, 32, 48, 48])]
+    ens = st_serve.Ensemble(runs, autotune=1)
+    try:
+        ens.march(3)
+        got = ens.results()
+        tuned = dict(ens.tuned)
+    finally:
+        ens.close()
+    _log("tuned 48", {k: v for k, v in tuned[48].items() if k.split(".")[0] in ("grad", "grad_cross", "hdiv_rkf")})
+    _, ref = st_serve.plain_one_at_a_time(runs, configs=tuned)
+    for r in runs:
+        assert np.array_equal(got[r.rid], ref[r.rid]), r.rid
+    s = st_amd.Solver(48, 48, 48, Re=200.0)
+    try:
+        s.set_pass_configs(tuned[48])
+        assert s.pass_configs() == tuned[48]
+    finally:
+        s.close()
