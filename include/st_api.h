@@ -84,8 +84,23 @@ int st_rhs(st_ctx *ctx, const double *Q, double *Qrhs);
 /* Hyper_RHS(Q, Qrhs)  src/hyper_rhs.f:12: adds the artificial-transport
  * terms to the right-hand side of the preceding st_rhs (same Q). */
 int st_hyper_rhs(st_ctx *ctx, double *Qrhs);
-/* Compute_hyper(Q) coefficients after st_hyper_rhs: beta*, mu*, kappa*
- * (src/hyper_dynamic.f:173, /HYPER/). */
+/* Compute_hyper(Q)  src/hyper_dynamic.f:173-318 (non-periodic x: the static
+ * coefficients of src/hyper_routines.f:66-214), as Program Main calls it
+ * before the first time step (src/main.f:100): beta*, mu*, kappa* (/HYPER/)
+ * from the state and the velocity gradients /GRAD/ as the last st_rhs or
+ * st_compute_grad left them (zero on a new context, as the reference's
+ * /GRAD/ is at start-up).  Q == NULL uses the resident state; otherwise Q
+ * is uploaded as the resident state first (as st_rhs does).  The next
+ * st_tstep's dt limits read these coefficients.  st_slot_load does the
+ * same for its slot with zero gradients. */
+int st_compute_hyper(st_ctx *ctx, const double *Q);
+/* Compute_Grad  src/grad.f:13-115 (called by Output, src/out.f:718), on the
+ * resident state: overwrites the nine velocity gradients of /GRAD/ and
+ * returns MaxDil = max |div u| and Enstrophy = (1/2) <|curl u|^2>, the log
+ * columns 6 and 7 of src/main.f:110-116.  Either pointer may be NULL. */
+int st_compute_grad(st_ctx *ctx, double *maxdil, double *enstrophy);
+/* Compute_hyper(Q) coefficients after st_hyper_rhs or st_compute_hyper:
+ * beta*, mu*, kappa* (src/hyper_dynamic.f:173, /HYPER/). */
 int st_get_hyper(st_ctx *ctx, double *H3);
 int st_set_hyper(st_ctx *ctx, const double *H3);
 /* /GRAD/ after st_rhs: dux_dx duy_dx duz_dx dux_dy duy_dy duz_dy dux_dz

@@ -69,6 +69,9 @@ def _load():
     lib.or_march.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     lib.or_march.restype = ctypes.c_double
     lib.or_diagnostics.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.or_compute_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    lib.or_clamps.argtypes = [ctypes.c_void_p]
+    lib.or_clamps.restype = ctypes.c_long
     lib.or_rk_coeffs.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.or_rk_coeffs.restype = ctypes.c_int
     return lib
@@ -271,6 +274,17 @@ class Oracle:
             if log is not None:
                 log.append(self.diagnostics(Qw))
         return Qw, t[0], s.value, dts
+
+    def clamps(self):
+        """Points Fix_Pressure has clamped (P < 0 -> base_pressure) on this context."""
+        return lib().or_clamps(self.h)
+
+    def compute_grad(self, Q):
+        """Compute_Grad (src/grad.f:13-115): (MaxDil, Enstrophy); /GRAD/ velocity part updated."""
+        Q = np.asfortranarray(Q)
+        m, e = np.zeros(1), np.zeros(1)
+        lib().or_compute_grad(self.h, _p(Q), _p(m), _p(e))
+        return m[0], e[0]
 
     def diagnostics(self, Q):
         Q = np.asfortranarray(Q)

@@ -261,6 +261,7 @@ struct or_ctx {
     double *hyp;                /* /HYPER/: beta_hyper, visc_hyper, k_hyper */
     double *arena;              /* persistent scratch fields (first-touched in parallel) */
     long arena_fields, arena_top;
+    long clamps;                /* points Fix_Pressure clamped since creation (a probe for tests) */
 };
 
 /* Scratch fields come from a persistent arena used as a stack, so repeated
@@ -1309,13 +1310,21 @@ void or_fix_pressure(or_ctx *c, double *Q)
     const double fac1 = c->p.gamma - 1.0;
     const double base_pressure = 0.71463;
     double *Q1 = FLD(Q, 0), *Q2 = FLD(Q, 1), *Q3 = FLD(Q, 2), *Q4 = FLD(Q, 3), *Q5 = FLD(Q, 4);
-    PFOR {
+    long nclamp = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nclamp)
+    for (long l = 0; l < N; l++) {
         const double Vs = 1.0 / Q4[l];
         double P = fac1 * (Q5[l] - 5.0e-1 * Vs * (Q1[l] * Q1[l] + Q2[l] * Q2[l] + Q3[l] * Q3[l]));
-        if (P < 0.0) P = base_pressure;
+        if (P < 0.0) {
+            P = base_pressure;
+            nclamp++;
+        }
         Q5[l] = (1.0 / fac1) * P + 5.0e-1 * Vs * (Q1[l] * Q1[l] + Q2[l] * Q2[l] + Q3[l] * Q3[l]);
     }
+    c->clamps += nclamp;
 }
+
+long or_clamps(const or_ctx *c) { return c->clamps; }
 
 /* Compute_CFL :7-65 */
 void or_compute_cfl(or_ctx *c, const double *Q, double *stab, double *resv)
@@ -1479,6 +1488,50 @@ static double compute_sum(const or_ctx *c, const double *F)
 }
 
 /* Compute_Mom/KE/Mass/Energy/Entropy :153-302 */
+/* Compute_Grad src/grad.f:13-115: the nine velocity gradients of /GRAD/
+   from Q, MaxDil = max |dil| (Compute_Max) and Enstrophy = 0.5 sum
+   |omega|^2 / (Nx Ny Nz) (Compute_Sum, src/auxiliary.f:187-216) */
+void or_compute_grad(or_ctx *c, const double *Q, double *maxdil, double *enstrophy)
+{
+    const long N = c->npts;
+    const double *Q1 = FLD(Q, 0), *Q2 = FLD(Q, 1), *Q3 = FLD(Q, 2), *Q4 = FLD(Q, 3);
+    double *G = c->grad;
+    double *dux_dx = G, *duy_dx = G + N, *duz_dx = G + 2 * N;
+    double *dux_dy = G + 3 * N, *duy_dy = G + 4 * N, *duz_dy = G + 5 * N;
+    double *dux_dz = G + 6 * N, *duy_dz = G + 7 * N, *duz_dz = G + 8 * N;
+    const long mark = c->arena_top;
+    double *U = ws_get(c, 3), *Gr = ws_get(c, 4), *W = ws_get(c, 1);
+    PFOR {                                                   /* :40-48 */
+        const double tmp = 1.0 / Q4[l];
+        U[l] = Q1[l] * tmp;
+        U[N + l] = Q2[l] * tmp;
+        U[2 * N + l] = Q3[l] * tmp;
+    }
+    or_dfd(c, 1, U + 2 * N, duz_dy, 0, 1.0);                 /* :55-57 */
+    or_dfd(c, 2, U + N, duy_dz, 0, 1.0);
+    PFOR Gr[l] = duz_dy[l] - duy_dz[l];
+    or_dfd(c, 0, U + 2 * N, duz_dx, 0, 1.0);                 /* :66-68 */
+    or_dfd(c, 2, U, dux_dz, 0, 1.0);
+    PFOR Gr[N + l] = dux_dz[l] - duz_dx[l];
+    or_dfd(c, 0, U + N, duy_dx, 0, 1.0);                     /* :77-79 */
+    or_dfd(c, 1, U, dux_dy, 0, 1.0);
+    PFOR Gr[2 * N + l] = duy_dx[l] - dux_dy[l];
+    or_dfd(c, 0, U, dux_dx, 0, 1.0);                         /* :88-96 */
+    or_dfd(c, 1, U + N, duy_dy, 0, 1.0);
+    or_dfd(c, 2, U + 2 * N, duz_dz, 0, 1.0);
+    PFOR Gr[3 * N + l] = dux_dx[l] + duy_dy[l] + duz_dz[l];
+    double m = 0.0;                                          /* :100-104 */
+    for (long l = 0; l < N; l++) {
+        const double a = fabs(Gr[3 * N + l]);
+        if (a > m) m = a;
+    }
+    PFOR W[l] = Gr[l] * Gr[l] + Gr[N + l] * Gr[N + l] + Gr[2 * N + l] * Gr[2 * N + l];   /* :110-111 */
+    const double NN = (double)(c->n[0] * c->n[1] * c->n[2]);
+    if (maxdil) *maxdil = m;
+    if (enstrophy) *enstrophy = 0.5 * compute_sum(c, W) / NN;   /* :113-115 */
+    c->arena_top = mark;
+}
+
 void or_diagnostics(or_ctx *c, const double *Q, double *out)
 {
     const long N = c->npts;

@@ -75,6 +75,8 @@ void or_set_hyper(or_ctx *c, const double *in3);
 
 /* time integration (src/lddrk4_periodic_sf.f) */
 void or_fix_pressure(or_ctx *c, double *Q);
+/* points Fix_Pressure has clamped on this context (test probe) */
+long or_clamps(const or_ctx *c);
 void or_compute_cfl(or_ctx *c, const double *Q, double *stab, double *resv);
 double or_compute_tstep(or_ctx *c, const double *Q, double cfl_resv, int step);
 void or_rk_stage(or_ctx *c, double *Q, double *QW, double *dQ, double alpha, double beta, double dt);
@@ -84,6 +86,8 @@ void or_set_sponge(or_ctx *c, double amp, double power, double frac, const doubl
 void or_lddrk4(or_ctx *c, double *Q, double dt, double *time, int *step);
 /* one march-mode-1 iteration of src/main.f:129-136: CFL, dt, LDDRK4 */
 double or_march(or_ctx *c, double *Q, double *time, int *step);
+/* Compute_Grad (src/grad.f:13-115): velocity part of /GRAD/, MaxDil, Enstrophy */
+void or_compute_grad(or_ctx *c, const double *Q, double *maxdil, double *enstrophy);
 /* MomX, MomY, MomZ, KE, Mass, Energy, Entropy (src/lddrk4_periodic_sf.f:153-302) */
 void or_diagnostics(or_ctx *c, const double *Q, double *out7);
 

@@ -225,13 +225,6 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
 // ---------------------------------------------------------------------
 // creation
 // ---------------------------------------------------------------------
-// src/define_periodic_sf.f:42-44; a non-periodic direction spans its period
-// with both end points (src/define_nonperiodic1d_sf.f:42)
-static double leng_of(double period, int n, int nonper = 0)
-{
-    return nonper ? period : period * (double)(n - 1) / (double)n;
-}
-
 extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int nslots, st_ctx **out)
 {
     if (!grid || !params || !out || nslots < 1) return -1;
@@ -311,12 +304,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     c->ph.ck = 1.0e-2;
     c->ph.cfl_cal = params->cfl_cal;
     const int np[3] = {grid->nonperiodic_x ? 1 : 0, 0, 0};
-    for (int d = 0; d < 3; d++) {
-        const double leng = leng_of(period[d], n[d], np[d]);
-        const double g1 = start[d] + leng * 0.0 / (double)(n[d] - 1);
-        const double g2 = start[d] + leng * 1.0 / (double)(n[d] - 1);
-        c->ph.D[d] = g2 - g1;                                  // src/grid.f:40
-    }
+    for (int d = 0; d < 3; d++) c->ph.D[d] = st::grid_spacing(period[d], start[d], n[d], np[d]);   // src/grid.f:40
     c->ph.dx6 = std::pow(c->ph.D[0], 6.0);
     c->ph.dx5 = std::pow(c->ph.D[0], 5.0);
     // scheme coefficients (src/header:193-203, src/LES_header:107-113,153-157)
@@ -346,7 +334,7 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     for (int d = 0; d < 3; d++) {
         int M, P;
         st::choose_layout(n[d], &M, &P);
-        const double sx1 = 1.0 / leng_of(period[d], n[d], np[d]);   // src/grid.f:31-32
+        const double sx1 = 1.0 / st::grid_leng(period[d], n[d], np[d]);   // src/grid.f:31-32
         const double sx2 = sx1 * sx1;
         c->sx1[d] = sx1;
         // the narrow layout (M/2 points per lane, 2P lanes) where it exists;
@@ -449,7 +437,24 @@ extern "C" void st_ctx_destroy(st_ctx *c)
     delete c;
 }
 
-extern "C" const char *st_last_error(const st_ctx *c) { return c ? c->err : "null context"; }
+namespace {
+thread_local char call_err[512];          // last failure of a call without a context
+}
+
+int st::set_call_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(call_err, sizeof(call_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *st_last_error(const st_ctx *c)
+{
+    if (c) return c->err;
+    return call_err[0] ? call_err : "null context";
+}
 
 extern "C" int st_sync(st_ctx *c)
 {
@@ -632,6 +637,83 @@ int reduce(st_ctx *c, int which, int nv, double *out)
     return 0;
 }
 
+int kerr(st_ctx *c, hipError_t e, const char *what)
+{
+    if (e != hipSuccess) return fail(c, -4, "%s: %s", what, hipGetErrorString(e));
+    return 0;
+}
+
+// One single-operator pass over one slot's fields (the pointers carry the
+// slot's offset; the launch covers slot 0 of a one-slot argument set).
+int op1(st_ctx *c, int kind, int dir, const double *in, double *out0, double *out1, int iadd, double cadd)
+{
+    st::PassArgs a = base_args(c);
+    a.in0 = in;
+    a.out0 = out0;
+    a.out1 = out1;
+    a.iadd = iadd;
+    return pass(c, kind, dir, a, cadd);
+}
+
+// Compute_Laplacian (src/hyper_dynamic.f:41-58): out = D2z + D2x + D2y of f
+// (|.| with absval, :278-279); tmp holds 3 fields.
+int laplacian(st_ctx *c, const double *f, double *out, double *tmp, int absval)
+{
+    const long N = c->N;
+    TRY(op1(c, st::K_D2, 0, f, nullptr, tmp, 0, 1.0));
+    TRY(op1(c, st::K_D2, 1, f, nullptr, tmp + N, 0, 1.0));
+    TRY(op1(c, st::K_D2, 2, f, nullptr, tmp + 2 * N, 0, 1.0));
+    return kerr(c, st::run_sum3(tmp + 2 * N, tmp, tmp + N, c->ni, absval, out, c->s), "laplacian sum");
+}
+
+// Compute_hyper (src/hyper_dynamic.f:173-318; with a non-periodic x the
+// static coefficients of src/hyper_routines.f:66-214) for one slot, outside
+// a stage: beta*, mu*, kappa* into Hc from the resident state and, with
+// use_grad, the velocity gradients in G (/GRAD/ as the last RHS left it);
+// without, |S| = 0, as at a start before any RHS (src/main.f:100 with
+// /GRAD/ still zero).  Scratch: the 10 partial-sum fields and T5, which a
+// stage rewrites before reading.
+int seq_compute_hyper(st_ctx *c, int slot, int use_grad)
+{
+    const long off = (long)slot * c->ni, N = c->N, ni = c->ni;
+    const double *Q = c->Q + off;
+    double *W = c->dQp + off;
+    double *T = c->T5 + off;
+    TRY(kerr(c, st::run_hyp_prep(Q, use_grad ? c->G + off : nullptr, N, ni, 1, W, c->s), "hyper prep"));
+    double *Cb = nullptr;
+    if (!c->g.nonperiodic_x) {
+        // Compute_coeff1 (:72-165): vorticity W[3..5] and dilatation W[6] of U
+        double *U0 = W, *U1 = W + N, *U2 = W + 2 * N;
+        double *wx = W + 3 * N, *wy = W + 4 * N, *wz = W + 5 * N, *dil = W + 6 * N;
+        TRY(op1(c, st::K_D1, 1, U2, wx, nullptr, 0, 1.0));            // :105-106
+        TRY(op1(c, st::K_D1, 2, U1, wx, nullptr, 1, -1.0));
+        TRY(op1(c, st::K_D1, 0, U2, wy, nullptr, 0, -1.0));           // :114-115
+        TRY(op1(c, st::K_D1, 2, U0, wy, nullptr, 1, 1.0));
+        TRY(op1(c, st::K_D1, 0, U1, wz, nullptr, 0, 1.0));            // :123-124
+        TRY(op1(c, st::K_D1, 1, U0, wz, nullptr, 1, -1.0));
+        TRY(op1(c, st::K_D1, 0, U0, dil, nullptr, 0, 1.0));           // :132-140
+        TRY(op1(c, st::K_D1, 1, U1, dil, nullptr, 1, 1.0));
+        TRY(op1(c, st::K_D1, 2, U2, dil, nullptr, 1, 1.0));
+        Cb = c->Cb + off;
+        TRY(kerr(c, st::run_hyp_cb(W, N, ni, Cb, c->s), "C_beta"));
+    }
+    // |Lap^2 S| into W[7], |Lap^2 e| into W[8] (:272-279)
+    TRY(laplacian(c, W + 7 * N, W + 9 * N, T, 0));
+    TRY(laplacian(c, W + 9 * N, W + 7 * N, T, 1));
+    TRY(laplacian(c, W + 8 * N, W + 9 * N, T, 0));
+    TRY(laplacian(c, W + 9 * N, W + 8 * N, T, 1));
+    // eta (:284-297), Gaussian x, y, z (:300), the coefficients (:303-314)
+    double *eta = c->eta + off;
+    TRY(kerr(c, st::run_hyp_eta(Q, N, ni, c->ph, W + 7 * N, W + 8 * N, eta, c->s), "eta"));
+    TRY(op1(c, st::K_GAUSS, 0, eta, T, nullptr, 0, 1.0));
+    TRY(op1(c, st::K_GAUSS, 1, T, T + N, nullptr, 0, 1.0));
+    TRY(op1(c, st::K_GAUSS, 2, T + N, W + 9 * N, nullptr, 0, 1.0));
+    TRY(op1(c, st::K_GAUSS, 0, eta + N, T + 2 * N, nullptr, 0, 1.0));
+    TRY(op1(c, st::K_GAUSS, 1, T + 2 * N, T + 3 * N, nullptr, 0, 1.0));
+    TRY(op1(c, st::K_GAUSS, 2, T + 3 * N, W + 7 * N, nullptr, 0, 1.0));
+    return kerr(c, st::run_hyp_coef(W + 9 * N, W + 7 * N, Cb, N, ni, c->ph, c->Hc + off, c->s), "coefficients");
+}
+
 int stage_loop(st_ctx *c, double dt, double *time, int *step)
 {
     const st::PassArgs base = base_args(c);
@@ -807,6 +889,40 @@ extern "C" int st_hyper_rhs(st_ctx *c, double *Qrhs)
     return 0;
 }
 
+extern "C" int st_compute_hyper(st_ctx *c, const double *Q)
+{
+    if (!c) return -1;
+    if (Q) TRY(up(c, c->Q, Q, 5));
+    if (c->p.hyper_param) TRY(seq_compute_hyper(c, 0, 1));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    return 0;
+}
+
+extern "C" int st_compute_grad(st_ctx *c, double *maxdil, double *enstrophy)
+{
+    if (!c) return -1;
+    const long N = c->N, ni = c->ni;
+    double *W = c->dQp;
+    TRY(kerr(c, st::run_hyp_prep(c->Q, nullptr, N, ni, 0, W, c->s), "grad prep"));
+    // /GRAD/ velocity gradients G[(3d + i) N] = d u_i / d x_d (src/grad.f:55-96)
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < 3; i++) TRY(op1(c, st::K_D1, d, W + i * N, c->G + (3 * d + i) * N, nullptr, 0, 1.0));
+    TRY(kerr(c, st::run_gradred(c->G, N, ni, c->red, c->s), "grad reductions"));
+    const int nb = st::gradred_blocks(ni);
+    HIPCHK(c, hipMemcpyAsync(c->red_h, c->red, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->prof) prof_collect(c);
+    double m = c->red_h[0], t = c->red_h[1];
+    for (int b = 1; b < nb; b++) {
+        m = c->red_h[2 * b] > m ? c->red_h[2 * b] : m;
+        t += c->red_h[2 * b + 1];
+    }
+    if (maxdil) *maxdil = m;                                               // Compute_Max (:104)
+    if (enstrophy) *enstrophy = 0.5 * t / ((double)c->Mx * c->My * c->Mz);   // :113-115
+    return 0;
+}
+
 extern "C" int st_get_hyper(st_ctx *c, double *H3)
 {
     if (!c || !H3) return -1;
@@ -955,17 +1071,21 @@ extern "C" int st_slot_load(st_ctx *c, int slot, const double *Q, double Re, dou
     TRY(slot_ok(c, slot));
     TRY(up(c, c->Q, Q, 5, slot));
     // the slot's clock and target, 1/Re (src/define_periodic_sf.f:179-189),
-    // and zero artificial-transport coefficients (no Compute_hyper yet)
+    // and the artificial-transport coefficients of a start: Compute_hyper
+    // before any RHS (src/main.f:100), so the dt limits of a slot resumed
+    // at step > 1 (or of the non-periodic rule) see kappa* of its state
     const double re_inv = Re > 0.0 ? 1.0 / Re : 0.0;
     const int target = step + nsteps;
     HIPCHK(c, hipMemcpyAsync(c->s_re + slot, &re_inv, sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipMemcpyAsync(c->s_time + slot, &time, sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipMemcpyAsync(c->s_step + slot, &step, sizeof(int), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipMemcpyAsync(c->s_target + slot, &target, sizeof(int), hipMemcpyHostToDevice, c->s));
-    for (int f = 0; f < 3; f++) {
-        hipError_t e = st::run_fill(c->Hc + f * c->N + (long)slot * c->ni, c->ni, 0.0, c->s);
-        if (e != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(e));
-    }
+    if (c->p.hyper_param) TRY(seq_compute_hyper(c, slot, 0));
+    else
+        for (int f = 0; f < 3; f++) {
+            hipError_t e = st::run_fill(c->Hc + f * c->N + (long)slot * c->ni, c->ni, 0.0, c->s);
+            if (e != hipSuccess) return fail(c, -4, "fill: %s", hipGetErrorString(e));
+        }
     HIPCHK(c, hipStreamSynchronize(c->s));
     c->act_h[slot] = 1;
     return push_act(c);

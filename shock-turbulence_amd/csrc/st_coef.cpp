@@ -234,6 +234,19 @@ int fill_np(OpTab *t, int op, double alpha, double beta, int n, int M, int P)
 
 }  // namespace
 
+double grid_leng(double period, int n, int nonper)
+{
+    return nonper ? period : period * (double)(n - 1) / (double)n;
+}
+
+double grid_spacing(double period, double start, int n, int nonper)
+{
+    const double leng = grid_leng(period, n, nonper);
+    const double g1 = start + leng * 0.0 / (double)(n - 1);
+    const double g2 = start + leng * 1.0 / (double)(n - 1);
+    return g2 - g1;
+}
+
 double sponge_sigma(int nx, int i, double amp, double power, double frac)
 {
     // src/sponge1d.f:33-36: CEILING((frac_sponge_x*Nx)/2D0) points at each end

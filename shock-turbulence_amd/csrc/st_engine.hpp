@@ -616,6 +616,19 @@ DEV Prim prim(const Phys &ph, double q1, double q2, double q3, double q4, double
     return r;
 }
 
+// Compute_CFL's per-point spectral radii (src/lddrk4_periodic_sf.f:29-49):
+// sum over directions of (c + |u_d|) k / Delta_d with k = 2.478 (w1, the
+// stability bound) and 2.199 (w2, the accuracy bound), c = sqrt(gamma (gamma-1) e).
+DEV void cfl_point(const Phys &ph, double q1, double q2, double q3, double q4, double q5, double &w1, double &w2)
+{
+    const double fac = ph.gamma * ph.gm1;
+    const double t1 = 1.0 / q4;
+    const double t2 = sqrt(fac * t1 * (q5 - 0.5 * t1 * (q1 * q1 + q2 * q2 + q3 * q3)));
+    const double ux = fabs(q1 * t1), uy = fabs(q2 * t1), uz = fabs(q3 * t1);
+    w1 = (t2 + ux) * 2.478 / ph.D[0] + (t2 + uy) * 2.478 / ph.D[1] + (t2 + uz) * 2.478 / ph.D[2];
+    w2 = (t2 + ux) * 2.199 / ph.D[0] + (t2 + uy) * 2.199 / ph.D[1] + (t2 + uz) * 2.199 / ph.D[2];
+}
+
 // mu = (gamma-1 T)^vis_exp / Re and d mu/dT = vis_exp mu / T
 // (src/rhsv_periodic_sf.f:245-247, the derivative written without a second
 // power; the two forms differ by rounding).

@@ -78,6 +78,17 @@ int build_dirtab(DirTab *t, int n, int M, int P, int B, double sx1, double sx2,
 // Choose (M, P) for a line length; returns 0 if n is unsupported.
 int choose_layout(int n, int *M, int *P);
 
+// Setup_Grid of one direction (src/grid.f:31-40, src/define_periodic_sf.f:42-44):
+// x_i = start + leng (i-1)/(n-1) with leng = period (n-1)/n, or the whole
+// period with both end points for a non-periodic direction
+// (src/define_nonperiodic1d_sf.f:42); the spacing is x_2 - x_1.
+double grid_leng(double period, int n, int nonper);
+double grid_spacing(double period, double start, int n, int nonper);
+
+// Failure of a call that has no context: records the message (per thread)
+// for st_last_error(NULL) and returns code.
+int set_call_error(int code, const char *fmt, ...);
+
 // Seeded isotropic turbulence initial condition (host).
 int ic_isotropic(int nx, int ny, int nz, const double period[3], double k0, double urms,
                  double mt, double gamma, unsigned long long seed, double *Q);

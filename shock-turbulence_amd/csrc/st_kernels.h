@@ -94,4 +94,16 @@ hipError_t run_dt(const double *pc, const double *pt, int nb, int nslot, const i
                   hipStream_t s);
 hipError_t run_clock(int nslot, const int *run, const double *dt, double *time, int *stepc, int bump, hipStream_t s);
 hipError_t run_sponge(const PassArgs &a, const double *spg, int Mx, int nleft, hipStream_t s);
+// pointwise steps of Compute_hyper and Compute_Grad outside a stage (one
+// slot; the pointers carry its offset, fields N apart)
+hipError_t run_hyp_prep(const double *Q, const double *G, long N, long ni, int want_hyp, double *W, hipStream_t s);
+hipError_t run_hyp_cb(const double *W, long N, long ni, double *Cb, hipStream_t s);
+hipError_t run_sum3(const double *z, const double *x, const double *y, long ni, int absval, double *out, hipStream_t s);
+hipError_t run_hyp_eta(const double *Q, long N, long ni, const Phys &ph, const double *d4S, const double *d4e,
+                       double *eta, hipStream_t s);
+hipError_t run_hyp_coef(const double *g1, const double *g2, const double *Cb, long N, long ni, const Phys &ph,
+                        double *Hc, hipStream_t s);
+// max |dil| and sum |omega|^2 partials (2 per block)
+int gradred_blocks(long ni);
+hipError_t run_gradred(const double *G, long N, long ni, double *partial, hipStream_t s);
 }  // namespace st

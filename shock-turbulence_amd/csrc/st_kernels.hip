@@ -98,17 +98,11 @@ __global__ void __launch_bounds__(256) k_cfl(PassArgs a, double *partial)
     const int slot = a.slots ? a.slots[blockIdx.y] : blockIdx.y;
     if (a.act && !a.act[slot]) return;
     const long N = a.N, off = (long)slot * a.ni;
-    const Phys &ph = a.ph;
-    const double fac = ph.gamma * ph.gm1;
     double v[2] = {-INFINITY, -INFINITY};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.ni; i += (long)gridDim.x * blockDim.x) {
         const long gi = off + i;
-        const double q1 = a.Q[gi], q2 = a.Q[N + gi], q3 = a.Q[2 * N + gi], q4 = a.Q[3 * N + gi], q5 = a.Q[4 * N + gi];
-        const double t1 = 1.0 / q4;
-        const double t2 = sqrt(fac * t1 * (q5 - 0.5 * t1 * (q1 * q1 + q2 * q2 + q3 * q3)));
-        const double ux = fabs(q1 * t1), uy = fabs(q2 * t1), uz = fabs(q3 * t1);
-        const double w1 = (t2 + ux) * 2.478 / ph.D[0] + (t2 + uy) * 2.478 / ph.D[1] + (t2 + uz) * 2.478 / ph.D[2];
-        const double w2 = (t2 + ux) * 2.199 / ph.D[0] + (t2 + uy) * 2.199 / ph.D[1] + (t2 + uz) * 2.199 / ph.D[2];
+        double w1, w2;
+        cfl_point(a.ph, a.Q[gi], a.Q[N + gi], a.Q[2 * N + gi], a.Q[3 * N + gi], a.Q[4 * N + gi], w1, w2);
         v[0] = w1 > v[0] ? w1 : v[0];
         v[1] = w2 > v[1] ? w2 : v[1];
     }
@@ -269,6 +263,175 @@ hipError_t run_sponge(const PassArgs &a, const double *spg, int Mx, int nleft, h
     long blocks = (a.ni + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_sponge, dim3((unsigned)blocks, (unsigned)a.nslot), dim3(256), 0, s, a, spg, Mx, nleft);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// Compute_hyper outside a stage (src/hyper_dynamic.f:173-318; the static
+// variant src/hyper_routines.f:66-214) and Compute_Grad (src/grad.f:13-115):
+// the pointwise steps between single-operator passes.  Each acts on one
+// slot: the pointers already carry the slot's offset, fields are N apart.
+// ---------------------------------------------------------------------
+
+// W[0..2] = Q_i * (1/rho) (src/hyper_dynamic.f:87-97, src/grad.f:40-48);
+// with want_hyp also W[7] = |S| of the symmetrised /GRAD/ (0 when G is
+// null: the coefficients of a start without gradients, src/main.f:100)
+// (src/hyper_dynamic.f:227-245, :12-33) and W[8] = e (:251-266).
+__global__ void __launch_bounds__(256) k_hyp_prep(const double *Q, const double *G, long N, long ni, int want_hyp,
+                                                  double *W)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        const double q1 = Q[i], q2 = Q[N + i], q3 = Q[2 * N + i], q4 = Q[3 * N + i];
+        const double t = 1.0 / q4;
+        W[i] = q1 * t;
+        W[N + i] = q2 * t;
+        W[2 * N + i] = q3 * t;
+        if (!want_hyp) continue;
+        double s = 0.0;
+        if (G) {
+            const double s11 = G[i], s22 = G[4 * N + i], s33 = G[8 * N + i];
+            const double s12 = 0.5 * (G[3 * N + i] + G[N + i]);
+            const double s13 = 0.5 * (G[6 * N + i] + G[2 * N + i]);
+            const double s23 = 0.5 * (G[7 * N + i] + G[5 * N + i]);
+            s = sqrt(0.5 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+        }
+        W[7 * N + i] = s;
+        W[8 * N + i] = t * (Q[4 * N + i] - 0.5 * t * (q1 * q1 + q2 * q2 + q3 * q3));
+    }
+}
+
+// C_beta from the vorticity W[3..5] and dilatation W[6]
+// (src/hyper_dynamic.f:142-160)
+__global__ void __launch_bounds__(256) k_hyp_cb(const double *W, long N, long ni, double *Cb)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        const double g1 = W[3 * N + i], g2 = W[4 * N + i], g3 = W[5 * N + i], g4 = W[6 * N + i];
+        const double fac = 0.5 * (1.0 - tanh(2.5 + 10.0 * g4));
+        const double om2 = g1 * g1 + g2 * g2 + g3 * g3;
+        Cb[i] = fac * g4 * g4 / (g4 * g4 + om2 + 1.0e-7) + 1.0e-20;
+    }
+}
+
+// Compute_Laplacian's sum d2S = D2z + D2x + D2y (src/hyper_dynamic.f:54),
+// optionally |.| (:278-279)
+__global__ void __launch_bounds__(256) k_sum3(const double *z, const double *x, const double *y, long ni, int absval,
+                                              double *out)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        const double s = z[i] + x[i] + y[i];
+        out[i] = absval ? fabs(s) : s;
+    }
+}
+
+// eta_1 = rho |Lap^2 S|, eta_2 = (rho/T) c |Lap^2 e| (src/hyper_dynamic.f:284-297)
+__global__ void __launch_bounds__(256) k_hyp_eta(const double *Q, long N, long ni, Phys ph, const double *d4S,
+                                                 const double *d4e, double *eta)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        const double q4 = Q[3 * N + i];
+        const Prim pr = prim(ph, Q[i], Q[N + i], Q[2 * N + i], q4, Q[4 * N + i]);
+        const double fac3 = (q4 / pr.T) * sqrt(ph.gamma * pr.p * pr.vs);
+        eta[i] = q4 * d4S[i];
+        eta[N + i] = fac3 * d4e[i];
+    }
+}
+
+// beta*, mu*, kappa* from the smoothed eta (src/hyper_dynamic.f:303-314;
+// C_beta = 1 without the sensor, src/hyper_routines.f:204)
+__global__ void __launch_bounds__(256) k_hyp_coef(const double *g1, const double *g2, const double *Cb, long N,
+                                                  long ni, Phys ph, double *Hc)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        Hc[i] = (Cb ? Cb[i] : 1.0) * g1[i] * ph.dx6;
+        Hc[N + i] = ph.cmu * g1[i] * ph.dx6;
+        Hc[2 * N + i] = ph.ck * g2[i] * ph.dx5;
+    }
+}
+
+// Compute_Grad's reductions (src/grad.f:100-115): max |dil| and the sum of
+// |omega|^2 over one slot, from /GRAD/ (G fields 0..8), partials per block.
+__global__ void __launch_bounds__(256) k_gradred(const double *G, long N, long ni, double *partial)
+{
+    __shared__ double sh[2][4];
+    double mx = 0.0, s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ni; i += (long)gridDim.x * blockDim.x) {
+        const double dux_dx = G[i], duy_dx = G[N + i], duz_dx = G[2 * N + i];
+        const double dux_dy = G[3 * N + i], duy_dy = G[4 * N + i], duz_dy = G[5 * N + i];
+        const double dux_dz = G[6 * N + i], duy_dz = G[7 * N + i], duz_dz = G[8 * N + i];
+        const double w1 = duz_dy - duy_dz, w2 = dux_dz - duz_dx, w3 = duy_dx - dux_dy;
+        const double dil = fabs(dux_dx + duy_dy + duz_dz);
+        mx = dil > mx ? dil : mx;
+        s += w1 * w1 + w2 * w2 + w3 * w3;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_xor(mx, o);
+        mx = a > mx ? a : mx;
+        s += __shfl_xor(s, o);
+    }
+    if (lane == 0) {
+        sh[0][wave] = mx;
+        sh[1][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sh[0][0], t = sh[1][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            m = sh[0][w] > m ? sh[0][w] : m;
+            t += sh[1][w];
+        }
+        partial[2 * blockIdx.x] = m;
+        partial[2 * blockIdx.x + 1] = t;
+    }
+}
+
+static long pw_blocks(long ni)
+{
+    long b = (ni + 255) / 256;
+    return b > 4096 ? 4096 : b;
+}
+
+hipError_t run_hyp_prep(const double *Q, const double *G, long N, long ni, int want_hyp, double *W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hyp_prep, dim3((unsigned)pw_blocks(ni)), dim3(256), 0, s, Q, G, N, ni, want_hyp, W);
+    return hipGetLastError();
+}
+
+hipError_t run_hyp_cb(const double *W, long N, long ni, double *Cb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hyp_cb, dim3((unsigned)pw_blocks(ni)), dim3(256), 0, s, W, N, ni, Cb);
+    return hipGetLastError();
+}
+
+hipError_t run_sum3(const double *z, const double *x, const double *y, long ni, int absval, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum3, dim3((unsigned)pw_blocks(ni)), dim3(256), 0, s, z, x, y, ni, absval, out);
+    return hipGetLastError();
+}
+
+hipError_t run_hyp_eta(const double *Q, long N, long ni, const Phys &ph, const double *d4S, const double *d4e,
+                       double *eta, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hyp_eta, dim3((unsigned)pw_blocks(ni)), dim3(256), 0, s, Q, N, ni, ph, d4S, d4e, eta);
+    return hipGetLastError();
+}
+
+hipError_t run_hyp_coef(const double *g1, const double *g2, const double *Cb, long N, long ni, const Phys &ph,
+                        double *Hc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hyp_coef, dim3((unsigned)pw_blocks(ni)), dim3(256), 0, s, g1, g2, Cb, N, ni, ph, Hc);
+    return hipGetLastError();
+}
+
+int gradred_blocks(long ni)
+{
+    long b = (ni + 255) / 256;
+    return (int)(b > 1024 ? 1024 : b);
+}
+
+hipError_t run_gradred(const double *G, long N, long ni, double *partial, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gradred, dim3((unsigned)gradred_blocks(ni)), dim3(256), 0, s, G, N, ni, partial);
     return hipGetLastError();
 }
 

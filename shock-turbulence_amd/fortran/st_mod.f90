@@ -120,6 +120,24 @@ module st_capi
        real(c_double), intent(inout) :: Qrhs(*)
      end function st_hyper_rhs
 
+     integer(c_int) function st_compute_hyper(ctx, Q) bind(C, name='st_compute_hyper')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: Q(*)
+     end function st_compute_hyper
+
+     integer(c_int) function st_get_hyper(ctx, H3) bind(C, name='st_get_hyper')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: H3(*)
+     end function st_get_hyper
+
+     integer(c_int) function st_compute_grad(ctx, maxdil, enstrophy) bind(C, name='st_compute_grad')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: maxdil, enstrophy
+     end function st_compute_grad
+
      integer(c_int) function st_fix_pressure(ctx, Q) bind(C, name='st_fix_pressure')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

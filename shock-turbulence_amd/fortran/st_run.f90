@@ -1,7 +1,10 @@
-! st_run.f90 -- march-mode-1 driver of the periodic solver on one MI355X,
-! following Program Main (src/main.f:99-152): set up, initial condition,
-! Compute_CFL / Compute_Tstep, then per step Compute_CFL, Compute_Tstep,
-! LDDRK4 and one line of the 'output' log.
+! st_run.f90 -- driver of the periodic solver on one MI355X, following
+! Program Main (src/main.f:59-184): set up, initial condition, Compute_hyper,
+! Compute_CFL / Compute_Tstep, then per step (march mode 1, constant CFL)
+! Compute_CFL, Compute_Tstep, LDDRK4, Compute_Grad (the part of Output the
+! log needs) and one line of the 'output' log; march mode 2 (constant time
+! step) keeps the start-up Tstep.  Modes 3 and 4 run to a wall-clock
+! budget and are refused.
 !
 ! Input: a namelist file named by the first argument (defaults below).
 ! The initial condition is the seeded isotropic field of st_ic_isotropic
@@ -19,11 +22,17 @@
 ! many steps (src/out.f:722-725; the reference writes every 50), in the
 ! format of Write_Restart (src/out.f:493-591).
 !
-! Output: the log columns of src/main.f:109-116 that this build computes
-! (MaxDil and Enstrophy of Compute_Grad are not), in the reference format
-! (1x,i5,ES13.5) appended to the file 'output', and the same values at full
-! precision on standard output.  With qout set, the final state is written
-! there as raw FP64 Q(My,Mz,Mx,5).
+! At start-up, as src/main.f:96-104: with out_init = 1, Compute_Grad fills
+! the velocity gradients of /GRAD/ first (Output(time, 0)); Compute_hyper
+! then sets the artificial-transport coefficients that the first dt limits
+! read (they bind from the first step after a restart at step > 1, and from
+! step 1 with a non-periodic x).
+!
+! Output: the 12 value columns of src/main.f:109-116 (n, t, dt, CFL_resv,
+! KE, MaxDil, Enstrophy, MomX, MomY, MomZ, Mass, Energy, Entropy) in the
+! reference's formats 10 and 11 (:316-317), appended to the file 'output',
+! and the same values at full precision on standard output.  With qout
+! set, the final state is written there as raw FP64 Q(My,Mz,Mx,5).
 program st_run
   use st_capi
   use st_state
@@ -34,14 +43,14 @@ program st_run
   real(8) :: gamma = 1.4d0, vis_exp = 0.76d0, bulk_ratio = 0.0d0, cfl_cal = 0.5d0
   real(8) :: period = 6.283185307179586d0
   character(len=256) :: qout = ' ', deck = ' ', restart_in = ' ', restart_dir = '.', prefix = 'st'
-  integer :: restart_every = 0
+  integer :: restart_every = 0, march_mode = 1, out_init = 0
   namelist /run/ nx, ny, nz, nsteps, hyper_param, d2_mode, rk_scheme, seed, k0, urms, mt, &
        re_lambda, Re, Pr, gamma, vis_exp, bulk_ratio, cfl_cal, period, qout, deck, restart_in, &
-       restart_dir, prefix, restart_every
+       restart_dir, prefix, restart_every, march_mode, out_init
 
   type(st_grid) :: grid
   type(st_params) :: params
-  real(8), allocatable :: Q(:, :, :, :)
+  real(8), allocatable, target :: Q(:, :, :, :)
   real(8) :: time, cfl_stab, cfl_resv, per(3), start(3)
   real(8) :: MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy
   integer :: step, step_count, u, ios, cue
@@ -73,6 +82,8 @@ program st_run
      hyper_param = dk%hyper_param
      cfl_cal = dk%cfl_cal
      nsteps = dk%Nsteps
+     march_mode = dk%march_mode
+     out_init = dk%out_init
      prefix = f_str(dk%prefix)
      restart_dir = f_str(dk%restart_path)
      if (len_trim(restart_in) == 0) then
@@ -82,6 +93,12 @@ program st_run
      end if
   else if (Re <= 0.0d0) then
      Re = (1.0d0 / (mt * mt)) ** vis_exp / (urms * (2.0d0 / k0) / re_lambda)
+  end if
+
+  if (march_mode /= 1 .and. march_mode /= 2) then
+     write(*, '(a,i0,a)') 'st_run: march_mode ', march_mode, &
+          ' (a wall-clock run_time budget, src/main.f:186-260) is not supported; use 1 or 2'
+     error stop 2
   end if
 
   grid = st_grid(nx, ny, nz, per(1), per(2), per(3), start(1), start(2), start(3))
@@ -99,21 +116,29 @@ program st_run
   end if
   call Upload_State(Q)
   cue = 1
+  call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
+  if (out_init == 1) call Compute_Grad()                 ! Output(time, 0), src/main.f:96
+  call Compute_hyper(Q)                                  ! src/main.f:100
   call Compute_CFL(Q, cfl_stab, cfl_resv, step)
   call Compute_Tstep(Q, cfl_resv, step)
-  call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
   call log_header()
   call log_line()
 
   do step_count = 1, nsteps
-     call Compute_CFL(Q, cfl_stab, cfl_resv, step)
-     call Compute_Tstep(Q, cfl_resv, step)
+     if (march_mode == 1) then                           ! src/main.f:129-136
+        call Compute_CFL(Q, cfl_stab, cfl_resv, step)
+        call Compute_Tstep(Q, cfl_resv, step)
+     else                                                ! src/main.f:158-168: the start-up Tstep
+        call Compute_CFL(Q, cfl_stab, cfl_resv, step)
+        cfl_stab = cfl_stab * Tstep
+        cfl_resv = cfl_resv * Tstep
+     end if
      call LDDRK4(time, step)
+     call Compute_Grad()                                 ! Output(time, step), src/out.f:718
      call Compute_Diagnostics(MomX, MomY, MomZ, KE, Mass_t, e_t, Entropy)
      call log_line()
      if (restart_every > 0) then
         if (mod(step, restart_every) == 0) then
-           call Download_State(Q)
            write(fname, '(a,a,a,a,i0)') trim(restart_dir), '/', trim(prefix), '.restart.', cue
            call st_check(st_restart_write(c_str(fname), nx, ny, nz, Q, time, step), &
                 'st_restart_write ' // trim(fname))
@@ -123,7 +148,6 @@ program st_run
   end do
 
   if (len_trim(qout) > 0) then
-     call Download_State(Q)
      open(newunit=u, file=trim(qout), access='stream', form='unformatted', status='replace')
      write(u) Q
      close(u)
@@ -132,20 +156,21 @@ program st_run
 
 contains
 
+  ! formats 10 and 11 of src/main.f:316-317; the header of :109-112
   subroutine log_header()
     open(newunit=u, file='output', form='formatted', status='unknown', position='append')
-    write(u, '(1x,a5,10(a13))') 'n ', 't   ', 'dt    ', 'CFL_resv  ', 'KE     ', &
-         'MomX  ', 'MomY  ', 'MomZ  ', 'Mass  ', 'Energy ', 'Entropy'
+    write(u, '(1x,a5,12(a13))') 'n ', 't   ', 'dt    ', 'CFL_resv  ', 'KE     ', 'MaxDil  ', &
+         'Enstrophy ', 'MomX  ', 'MomY  ', 'MomZ  ', 'Mass  ', 'Energy ', 'Entropy'
     close(u)
   end subroutine log_header
 
   subroutine log_line()
     open(newunit=u, file='output', form='formatted', status='unknown', position='append')
-    write(u, '(1x,i5,10(ES13.5))') step, time, Tstep, cfl_resv * Tstep, KE, &
+    write(u, '(1x,i5,12(ES13.5))') step, time, Tstep, cfl_resv * Tstep, KE, MaxDil, Enstrophy, &
          MomX, MomY, MomZ, Mass_t, e_t, Entropy
     close(u)
-    write(*, '(a,i6,10(1x,ES24.16E3))') 'step', step, time, Tstep, cfl_resv * Tstep, KE, &
-         MomX, MomY, MomZ, Mass_t, e_t, Entropy
+    write(*, '(a,i6,12(1x,ES24.16E3))') 'step', step, time, Tstep, cfl_resv * Tstep, KE, MaxDil, &
+         Enstrophy, MomX, MomY, MomZ, Mass_t, e_t, Entropy
   end subroutine log_line
 
 end program st_run

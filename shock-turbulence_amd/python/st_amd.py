@@ -61,7 +61,8 @@ EXPORTS = [
     "st_rk_stage", "st_step", "st_march", "st_diagnostics", "st_profile", "st_profile_read", "st_pass_name",
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_pass_config_set", "st_query",
-    "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock",
+    "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock", "st_compute_hyper",
+    "st_compute_grad",
 ]
 
 _lib = None
@@ -95,6 +96,8 @@ def lib():
     L.st_rhs.argtypes = [vp, _dp, _dp]
     L.st_hyper_rhs.argtypes = [vp, _dp]
     L.st_get_hyper.argtypes = [vp, _dp]
+    L.st_compute_hyper.argtypes = [vp, _dp]
+    L.st_compute_grad.argtypes = [vp, _dp, _dp]
     L.st_set_hyper.argtypes = [vp, _dp]
     L.st_get_grad.argtypes = [vp, _dp]
     L.st_fix_pressure.argtypes = [vp, _dp]
@@ -323,6 +326,19 @@ class Solver:
         out = self._z(3)
         self._chk(lib().st_get_hyper(self.h, _p(out)), "st_get_hyper")
         return out
+
+    def compute_hyper(self, Q=None):
+        """Compute_hyper(Q) (src/main.f:100): the coefficients from the state
+        (Q, or the resident one) and the current /GRAD/; returns them."""
+        Qf = None if Q is None else _f(Q)
+        self._chk(lib().st_compute_hyper(self.h, _p(Qf)), "st_compute_hyper")
+        return self.hyper()
+
+    def compute_grad(self):
+        """Compute_Grad (src/grad.f:13-115) on the resident state: (MaxDil, Enstrophy)."""
+        m, e = np.zeros(1), np.zeros(1)
+        self._chk(lib().st_compute_grad(self.h, _p(m), _p(e)), "st_compute_grad")
+        return m[0], e[0]
 
     def set_hyper(self, H):
         H = _f(H)

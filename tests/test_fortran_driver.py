@@ -38,8 +38,14 @@ def test_driver_march_matches_oracle(tmp_path):
     lines = [l.split() for l in r.stdout.splitlines() if l.startswith("step")]
     assert len(lines) == nsteps + 1
     Qg = np.fromfile(tmp_path / "q.bin", dtype=np.float64).reshape((n, n, n, 5), order="F")
+    # the reference's log: a header and one line per step, 12 value columns
+    # after n in formats 10 and 11 (src/main.f:109-116, :316-317)
     log = (tmp_path / "output").read_text().splitlines()
-    assert len(log) == nsteps + 2 and log[0].split()[0] == "n"
+    assert len(log) == nsteps + 2
+    assert log[0].split() == ["n", "t", "dt", "CFL_resv", "KE", "MaxDil", "Enstrophy", "MomX", "MomY", "MomZ",
+                              "Mass", "Energy", "Entropy"]
+    assert all(len(l.split()) == 13 for l in log[1:])
+    assert len(log[1]) == 1 + 5 + 12 * 13
 
     re = st_amd.re_for_taylor(40.0, k0, urms, mt)
     o = O.Oracle(n, n, n, Re=re, Pr=0.7, hyper_param=1)
@@ -50,7 +56,46 @@ def test_driver_march_matches_oracle(tmp_path):
     assert int(lines[-1][1]) == st_c == nsteps
     assert abs(last[0] - t_c) <= 1e-12 * t_c
     assert abs(last[1] - dts[-1]) <= 1e-11 * dts[-1]
-    np.testing.assert_allclose(last[3:], o.diagnostics(Qc)[[3, 0, 1, 2, 4, 5, 6]], rtol=1e-10, atol=1e-13)
+    maxdil, ens = o.compute_grad(Qc)
+    d = o.diagnostics(Qc)
+    want = [d[3], maxdil, ens, d[0], d[1], d[2], d[4], d[5], d[6]]
+    np.testing.assert_allclose(last[3:], want, rtol=1e-10, atol=1e-13)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_driver_march_mode_2(tmp_path):
+    """Constant time step (src/main.f:158-184): the Tstep of the start-up
+    Compute_Tstep is kept, and the log's CFL column is cfl_resv Tstep^2."""
+    st_amd = pytest.importorskip("st_amd")
+    n, nsteps, seed = 32, 4, 77
+    nml = tmp_path / "run.nml"
+    nml.write_text("&run nx=%d, ny=%d, nz=%d, nsteps=%d, seed=%d, k0=4, urms=1, mt=0.3, re_lambda=40, pr=0.7, "
+                   "hyper_param=1, march_mode=2, qout='q.bin' /\n" % (n, n, n, nsteps, seed))
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [[float(x) for x in l.split()[1:]] for l in r.stdout.splitlines() if l.startswith("step")]
+    re = st_amd.re_for_taylor(40.0, 4.0, 1.0, 0.3)
+    o = O.Oracle(n, n, n, Re=re, Pr=0.7, hyper_param=1)
+    Q = st_amd.ic_isotropic(n, n, n, k0=4.0, urms=1.0, mt=0.3, seed=seed)
+    _, rv = o.cfl(Q)
+    dt = o.tstep(Q, rv, 0)
+    t = 0.0
+    for k in range(nsteps):
+        _, rv = o.cfl(Q)
+        Q, t, _ = o.lddrk4(Q, dt, time=t, step=k)
+        assert abs(lines[k + 1][2] - dt) == 0.0
+        assert abs(lines[k + 1][3] - rv * dt * dt) <= 1e-11 * rv * dt * dt
+    Qg = np.fromfile(tmp_path / "q.bin", dtype=np.float64).reshape((n, n, n, 5), order="F")
+    assert relerr(Qg, Q).max() < 1e-10
+
+
+@needs_exe
+def test_driver_rejects_wall_clock_modes(tmp_path):
+    nml = tmp_path / "run.nml"
+    nml.write_text("&run nx=32, ny=32, nz=32, nsteps=2, march_mode=3 /\n")
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "march_mode 3" in r.stdout
 
 
 @needs_exe
