@@ -111,6 +111,18 @@ int st_get_grad(st_ctx *ctx, double *G12);
 int st_fix_pressure(st_ctx *ctx, double *Q);
 /* Compute_CFL  src/lddrk4_periodic_sf.f:7, on the resident state. */
 int st_cfl(st_ctx *ctx, double *cfl_stab, double *cfl_resv);
+/* Compute_CFL that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Q_dev is
+ * a device array Q(My,Mz,Mx,5) of the given grid; out_dev (device, 2
+ * doubles) receives cfl_stab and cfl_resv, the same bits as st_cfl on that
+ * state.  The work is enqueued on `stream` (a hipStream_t; NULL: the null
+ * stream) and not waited for.  No context: the per-block partials live in
+ * a fixed 256 KiB device array of the library, one per GPU, so grids of at
+ * most 33,554,432 points are accepted (larger ones return -2 before any
+ * launch); calls that overlap on one GPU race on that array.  Errors:
+ * st_last_error(NULL). */
+int st_cfl_noalloc(const st_grid *grid, double gamma, const double *Q_dev, double *out_dev, void *stream);
 /* Compute_Tstep  src/lddrk4_periodic_sf.f:74 (hyper limits when step > 1). */
 int st_tstep(st_ctx *ctx, double cfl_resv, int step, double *dt);
 /* One stage of LDDRK4 (src/lddrk4_periodic_sf.f:344-356) on the resident

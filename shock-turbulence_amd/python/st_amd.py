@@ -62,7 +62,7 @@ EXPORTS = [
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_pass_config_set", "st_query",
     "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock", "st_compute_hyper",
-    "st_compute_grad",
+    "st_compute_grad", "st_cfl_noalloc",
 ]
 
 _lib = None
@@ -98,6 +98,7 @@ def lib():
     L.st_get_hyper.argtypes = [vp, _dp]
     L.st_compute_hyper.argtypes = [vp, _dp]
     L.st_compute_grad.argtypes = [vp, _dp, _dp]
+    L.st_cfl_noalloc.argtypes = [ctypes.POINTER(Grid), ctypes.c_double, vp, vp, vp]
     L.st_set_hyper.argtypes = [vp, _dp]
     L.st_get_grad.argtypes = [vp, _dp]
     L.st_fix_pressure.argtypes = [vp, _dp]
