@@ -20,6 +20,7 @@ single GPU.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -94,32 +95,57 @@ def copy_bandwidth(nbytes=1 << 31, reps=5):
     return round(best, 1)
 
 
-def cpu_baseline(n, threads, Q, kw, min_s=10.0):
-    """Time the CPU oracle (C restatement, OpenMP) on the first RK stages of
-    the same workload's first step, until min_s seconds have passed (at most
-    the 5 stages of the first RK loop)."""
+def host_cpu():
+    """nproc, the CPU model name (lscpu / /proc/cpuinfo) and the threads this
+    process may use (its CPU affinity and OMP_NUM_THREADS)."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    granted = min(avail, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else avail
+    return {"nproc": nproc, "affinity_cpus": avail, "model": model, "threads": granted}
+
+
+def cpu_baseline(n, threads, Q, kw):
+    """Time the CPU oracle (C restatement, OpenMP) over one full step of the
+    same workload (CFL, dt, the 11 RK stages) from the same initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
+    host = host_cpu()
+    if threads <= 0:
+        threads = host["threads"]
     O.lib().or_threads(threads)
     o = O.Oracle(n, n, n, **kw)
-    stab, resv = o.cfl(Q)
-    dt = o.tstep(Q, resv, 0)
-    ra, rb = O.rk_coeffs(0, 0)
-    QW = np.zeros_like(Q)
-    k = 0
     t0 = time.perf_counter()
-    while k < len(ra):
-        Q, QW, _ = o.rk_stage(Q, QW, ra[k], rb[k], dt)
-        k += 1
-        if time.perf_counter() - t0 >= min_s:
-            break
+    o.march(Q, 1)
     el = time.perf_counter() - t0
     pts = float(n) ** 3
-    return {"value": round(pts * k / el / 1e6, 3), "unit": "Mpts*stage/s", "cores": threads, "kind": "port",
-            "sample": "the first %d RK stages (RHS + Hyper_RHS + update + filter + Fix_Pressure each) of the first "
-                      "step at %d^3 from the same initial state, oracle/st_oracle.c with %d OpenMP threads, "
-                      "%.2f s" % (k, n, threads, el)}
+    return {"value": round(pts * STAGES_PER_STEP / el / 1e6, 3), "unit": "Mpts*stage/s", "cores": threads,
+            "kind": "port", "nproc": host["nproc"], "cpu_model": host["model"],
+            "sample": "one full step (Compute_CFL, Compute_Tstep, the %d RK stages: RHS + Hyper_RHS + update + "
+                      "filter + Fix_Pressure each) at %d^3 from the same initial state, oracle/st_oracle.c with %d "
+                      "OpenMP threads (the CPUs this process may use: %d of nproc %d), %.2f s"
+                      % (STAGES_PER_STEP, n, threads, host["affinity_cpus"], host["nproc"], el)}
 
 
 def ensemble_leg(nsteps, autotune):
@@ -225,10 +251,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads for the CPU baseline (0: every CPU this process may use)")
     ap.add_argument("--profile", type=int, default=1, help="per-pass HIP-event timing (in extra steps after the timed region)")
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--ensemble-steps", type=int, default=6, help="batched-ensemble leg (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg, steps per instance (0: skip)")
     ap.add_argument("--autotune", type=int, default=2,
                     help="ensemble and serving legs: tune each batch's launch shapes at set-up (stages per "
                          "candidate; the plain path gets the same shapes); 0: the 256^3 defaults")
