@@ -160,11 +160,21 @@ def ensemble_leg(nsteps, autotune):
     ens = st_serve.Ensemble(runs, autotune=autotune)
     tuned = dict(ens.tuned)
     ens.march(0)
+    # each size's batch on its own for the first few steps (its batched rate
+    # alone), then all batches together for the rest (the mix)
+    k_alone = min(5, nsteps - 1)
+    alone = {}
+    for n, p in ens.pools.items():
+        t0 = time.perf_counter()
+        p.batch.march(k_alone)
+        p.batch.sync()
+        alone[n] = len(p.running) * n ** 3 * STAGES_PER_STEP * k_alone / (time.perf_counter() - t0) / 1e6
     t0 = time.perf_counter()
-    ens.march(nsteps)
+    ens.march(nsteps - k_alone)
     el = time.perf_counter() - t0
     ens.close()
-    plain = 0.0
+    mix_rate = (work * (nsteps - k_alone) / nsteps) / el / 1e6
+    plain_t = {}
     for r in runs:
         s = st_amd.Solver(r.n, r.n, r.n, Re=r.Re, Pr=0.7, hyper_param=1, cfl_cal=0.5)
         if r.n in tuned:
@@ -174,14 +184,29 @@ def ensemble_leg(nsteps, autotune):
         t0 = time.perf_counter()
         s.march(nsteps)
         s.sync()
-        plain += time.perf_counter() - t0
+        plain_t[r.n] = plain_t.get(r.n, 0.0) + time.perf_counter() - t0
         s.close()
+    plain = sum(plain_t.values())
+    plain_rate = work / plain / 1e6
+    per_size = {}
+    for n in sorted(plain_t):
+        w = sum(r.n ** 3 * STAGES_PER_STEP * nsteps for r in runs if r.n == n)
+        per_size[str(n)] = {"runs": sum(1 for r in runs if r.n == n), "work_share": round(w / work, 4),
+                            "plain_mpts_stage_per_s": round(w / plain_t[n] / 1e6, 1),
+                            "batched_alone_mpts_stage_per_s": round(alone[n], 1)}
     return {"workload": "48 instances: 16x48^3, 16x64^3, 12x96^3, 4x128^3, seeds 1..48, k0 in {3,4,6}, "
                         "Mt in {0.2,0.3,0.5}, Re_lambda=40; %d steps each" % nsteps,
             "points": sum(r.n ** 3 for r in runs), "steps": nsteps,
-            "batched_mpts_stage_per_s": round(work / el / 1e6, 1),
-            "plain_one_at_a_time_mpts_stage_per_s": round(work / plain / 1e6, 1),
-            "speedup": round(plain / el, 3), "autotune_reps": autotune}
+            "batched_mpts_stage_per_s": round(mix_rate, 1),
+            "plain_one_at_a_time_mpts_stage_per_s": round(plain_rate, 1),
+            "speedup": round(mix_rate / plain_rate, 3), "autotune_reps": autotune,
+            "per_size": per_size,
+            # what the one-at-a-time rate per size allows: a speedup of x
+            # over it needs the mix to run at x times the plain rate
+            "ceiling": {"rate_needed_for_2x_mpts_stage_per_s": round(2 * plain_rate, 1),
+                        "rate_needed_for_2x_frac_of_hbm_roofline": round(2 * plain_rate * B_ALG * 1e6 / 1e9
+                                                                         / HBM_PEAK_GBS, 3),
+                        "batched_mix_frac_of_hbm_roofline": round(mix_rate * B_ALG * 1e6 / 1e9 / HBM_PEAK_GBS, 3)}}
 
 
 def config2_leg(nsteps, warmup, autotune=3):
@@ -236,11 +261,18 @@ def serving_leg(nruns, rate, seed, autotune=0):
         srv.close()
     plain, ref = st_serve.plain_one_at_a_time(runs, configs=srv.tuned)
     same = all(bool((r.result == ref[r.rid]).all()) for r in runs)
+    pr = plain["mpts_stage_per_s"]
     return {"workload": "%d runs, Poisson arrivals at %.0f/s, N in {48,64,96,128}^3 uniform, steps U[50,400], "
                         "seed %d; HBM budget 200 GB" % (nruns, rate, seed),
             "served": {k: round(v, 4) if isinstance(v, float) else v for k, v in served.items()},
-            "plain_one_at_a_time": {k: round(v, 4) if isinstance(v, float) else v for k, v in plain.items()},
-            "throughput_ratio": round(served["mpts_stage_per_s"] / plain["mpts_stage_per_s"], 3),
+            "plain_one_at_a_time": {k: round(v, 4) if isinstance(v, float) else v for k, v in plain.items()
+                                    if k != "per_size"},
+            "plain_per_size": plain.get("per_size"),
+            "throughput_ratio": round(served["mpts_stage_per_s"] / pr, 3),
+            # a served throughput of 3x the plain path's needs this rate
+            "ceiling": {"rate_needed_for_3x_mpts_stage_per_s": round(3 * pr, 1),
+                        "rate_needed_for_3x_frac_of_hbm_roofline": round(3 * pr * B_ALG * 1e6 / 1e9
+                                                                         / HBM_PEAK_GBS, 3)},
             "results_equal_alone_and_in_mix": same, "autotune_reps": autotune}
 
 
@@ -396,6 +428,8 @@ def main():
         "nonperiodic_512x192x192": config2,
         "check": {"finite": finite, "sim_time": time_, "step": step, "dt_last": float(dts[-1]) if len(dts) else None,
                   "diagnostics": [float(x) for x in diag]},
+        # the library's source hash and the one of the sources in this tree
+        "build": {"library": st_amd.build_id(), "sources": st_amd.source_id()},
     }
     print(json.dumps(out))
 

@@ -54,6 +54,11 @@ typedef struct {
 int st_ctx_create(const st_grid *grid, const st_params *params, st_ctx **out);
 void st_ctx_destroy(st_ctx *ctx);
 const char *st_last_error(const st_ctx *ctx);
+/* Build provenance (no counterpart in the reference): "src-sha256:" and the
+ * first 16 hex digits of the SHA-256 of the library's sources (each file's
+ * path relative to the package, a newline, then its contents, in sorted
+ * path order), fixed at build time. */
+const char *st_build_id(void);
 int st_sync(st_ctx *ctx);
 
 /* State in / out (the /RKWORK/ Q array, src/main.f:31). */

@@ -450,6 +450,12 @@ int st::set_call_error(int code, const char *fmt, ...)
     return code;
 }
 
+#ifndef ST_SRC_HASH
+#define ST_SRC_HASH "unknown"
+#endif
+
+extern "C" const char *st_build_id(void) { return "src-sha256:" ST_SRC_HASH; }
+
 extern "C" const char *st_last_error(const st_ctx *c)
 {
     if (c) return c->err;

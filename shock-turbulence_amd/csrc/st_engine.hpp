@@ -46,15 +46,35 @@ DEV long tile_of(const Geo &g)
 
 // LDS layout of one slot: line b holds its n = M*P points in a block of
 // n doubles; point s = c*M + k sits at row k, column c.  Rows are rotated by
-// b and columns XOR-swizzled so that lane-parallel reads at fixed k and the
-// memory-order staging writes both spread over the 32 double-wide banks.
-// (M = 3 or 6: rows rotate by b mod M; any rotation and swizzle is a
-// bijection of the block, only the bank spread depends on them.)
+// b and columns XOR-swizzled by a key f(k, b).  The LDS serves 16 lanes of a
+// 64-bit access per cycle over 16 double-wide banks, so the key is chosen
+// such that every group of 16 lanes hits 16 distinct banks in all three
+// access patterns: the line-order reads and writes of the compute phase
+// (fixed k, consecutive c), the memory-order staging of a y tile
+// (consecutive s: 16/M chunks by M rows) and that of an x or z tile (B lines
+// side by side, 16/B points of each).  For P >= 16 (row offsets are then
+// multiples of 16) f = (16/M) k + g(b): the point-in-chunk bits sit above
+// the chunk bits of the y staging, and lds_g places the line's bits where
+// the x/z staging leaves the 16 banks free.  Narrower lines (P = 8) keep
+// f = k (P/M) + b (4b for M = 4).  (M = 3 or 6: rows rotate by b mod M; any
+// rotation and swizzle is a bijection of the block, only the bank spread
+// depends on them.)
 template <int M>
-HD int lds_idx(int b, int k, int c, int P)
+HD int lds_g(int b, int B)
+{
+    int lB = 0;
+    while ((1 << lB) < B) lB++;
+    constexpr int lM = M >= 8 ? 3 : M >= 4 ? 2 : M >= 2 ? 1 : 0;
+    if (lB + lM <= 4) return b << (4 - lB - lM);
+    const int lo = 4 - lM, kb = lB < 4 ? 4 - lB : 0;
+    return (b & ((1 << lo) - 1)) | ((b >> lo) << (lo + kb));
+}
+
+template <int M>
+HD int lds_idx(int b, int k, int c, int P, int g)
 {
     if constexpr ((M & (M - 1)) == 0) {
-        const int f = k * (P / M) + (M == 4 ? 4 * b : b);
+        const int f = P >= 16 ? (16 / M) * k + g : k * (P / M) + (M == 4 ? 4 * b : b);
         return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
     } else {
         int r = k + b % M;
@@ -122,11 +142,12 @@ struct Lane {
     static constexpr int P = P_;
     static constexpr bool NP = NP_;
     int c, seg, b;         // chunk index, first lane of the line, line in tile
+    int g;                 // the line's column key (lds_g)
     double *lds;
     int ss;                // slot size (doubles)
 
-    DEV double ld(int slot, int k) const { return lds[slot * ss + lds_idx<M>(b, k, c, P)]; }
-    DEV void st(int slot, int k, double v) const { lds[slot * ss + lds_idx<M>(b, k, c, P)] = v; }
+    DEV double ld(int slot, int k) const { return lds[slot * ss + lds_idx<M>(b, k, c, P, g)]; }
+    DEV void st(int slot, int k, double v) const { lds[slot * ss + lds_idx<M>(b, k, c, P, g)] = v; }
     DEV int src(int dc) const { return seg + ((c + dc) & (P - 1)); }
 };
 
@@ -141,6 +162,7 @@ DEV Lane<M, P, NP> make_lane(double *lds, int ss)
     L.c = lane & (P - 1);
     L.seg = lane - L.c;
     L.b = wave * (64 / P) + lane / P;
+    L.g = lds_g<M>(L.b, ss / (M * P));
     L.lds = lds;
     L.ss = ss;
     return L;

@@ -91,7 +91,7 @@ k_pass(Geo g, PassArgs a0)
         PASS::load(g, a, gi, v);
         if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, P);
+        const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
 #pragma unroll
         for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
     }
@@ -108,7 +108,7 @@ k_pass(Geo g, PassArgs a0)
         for (int r = 0; r < M; r++) {
             int b, s;
             const long gi = W.at(g, r, b, s);
-            const int li = lds_idx<M>(b, s % M, s / M, P);
+            const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NS];
 #pragma unroll
             for (int q = 0; q < NS; q++) v[q] = lds[q * ss + li];
@@ -127,7 +127,7 @@ k_pass(Geo g, PassArgs a0)
         for (int r = 0; r < M; r++) {
             int b, s;
             const long gi = W.at(g, r, b, s);
-            const int li = lds_idx<M>(b, s % M, s / M, P);
+            const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NE + NPF > 0 ? NE + NPF : 1];
 #pragma unroll
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
@@ -150,7 +150,7 @@ k_pass(Geo g, PassArgs a0)
         int b, s;
         const long gi = W.at(g, r, b, s);
         const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, P);
+        const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
         constexpr int NV = RE ? NO : NO + NE;
         constexpr int NK = keep_n<PASS>::value;
         static_assert(NK == 0 || (RE && NS - NK >= NO && NS - NK >= NE), "kept slots hold no results");

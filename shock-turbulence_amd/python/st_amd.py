@@ -62,7 +62,7 @@ EXPORTS = [
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_pass_config_set", "st_query",
     "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock", "st_compute_hyper",
-    "st_compute_grad", "st_cfl_noalloc",
+    "st_compute_grad", "st_cfl_noalloc", "st_build_id",
 ]
 
 _lib = None
@@ -83,6 +83,8 @@ def lib():
     L.st_ctx_destroy.restype = None
     L.st_last_error.argtypes = [vp]
     L.st_last_error.restype = ctypes.c_char_p
+    L.st_build_id.argtypes = []
+    L.st_build_id.restype = ctypes.c_char_p
     L.st_sync.argtypes = [vp]
     L.st_state_upload.argtypes = [vp, _dp]
     L.st_state_download.argtypes = [vp, _dp]
@@ -135,6 +137,31 @@ def lib():
     L.st_pass_config_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     _lib = L
     return L
+
+
+def source_id():
+    """The hash st_build_id() carries, recomputed from the source files in
+    this tree (the same file list, order and framing as the library's
+    Makefile)."""
+    import glob
+    import hashlib
+    csrc = os.path.join(_PKG, "csrc")
+    files = (glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.h"))
+             + glob.glob(os.path.join(csrc, "*.hpp"))
+             + [os.path.join(csrc, f) for f in ("st_api.cpp", "st_coef.cpp", "st_ic.cpp", "st_io.cpp")]
+             + [os.path.join(os.path.dirname(_PKG), "include", "st_api.h")])
+    root = os.path.dirname(_PKG)
+    h = hashlib.sha256()
+    for f in sorted(set(os.path.abspath(x) for x in files)):
+        h.update((os.path.relpath(f, root) + "\n").encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return "src-sha256:" + h.hexdigest()[:16]
+
+
+def build_id():
+    """The library's st_build_id()."""
+    return lib().st_build_id().decode()
 
 
 def _p(a):

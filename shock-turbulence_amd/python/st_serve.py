@@ -246,6 +246,16 @@ def plain_one_at_a_time(runs, configs=None, **kw):
         finally:
             s.close()
     total = sum(service.values())
+    per_size = {}
+    for r in runs:
+        d = per_size.setdefault(str(r.n), {"runs": 0, "work": 0.0, "busy_s": 0.0})
+        d["runs"] += 1
+        d["work"] += r.work
+        d["busy_s"] += service[r.rid]
+    for d in per_size.values():
+        d["mpts_stage_per_s"] = round(d["work"] / d["busy_s"] / 1e6, 1)
+        d["work_share"] = round(d.pop("work") / sum(r.work for r in runs), 4)
+        d["busy_s"] = round(d["busy_s"], 3)
     # first come, first served on one device
     clock = 0.0
     lat = []
@@ -262,6 +272,7 @@ def plain_one_at_a_time(runs, configs=None, **kw):
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
+        "per_size": per_size,
     }
     return st, results
 

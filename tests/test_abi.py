@@ -27,3 +27,9 @@ def test_ic_host_side():
     Q = st_amd.ic_isotropic(16, 16, 16, k0=3.0, urms=1.0, mt=0.3, seed=3)
     assert Q.shape == (16, 16, 16, 5)
     assert np.isfinite(Q).all()
+
+
+def test_library_built_from_these_sources():
+    """libst.so carries the hash of the sources it was built from; it must
+    be the hash of the sources in this tree (a stale library fails here)."""
+    assert st_amd.build_id() == st_amd.source_id(), (st_amd.build_id(), st_amd.source_id())
