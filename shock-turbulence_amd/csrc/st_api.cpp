@@ -275,8 +275,8 @@ extern "C" int st_batch_create(const st_grid *grid, const st_params *params, int
     for (int d = 0; d < 3; d++) {
         int M, P;
         if (n[d] < 16 || !st::choose_layout(n[d], &M, &P)) {
-            fail(c, -2, "unsupported line length %d in direction %d (need n = M*P, M in {2,4,6,8}, P in {8,16,32,64}: "
-                        "16..512 points, powers of two or three times one)", n[d], d);
+            fail(c, -2, "unsupported line length %d in direction %d (need n = M*P, M in {2,4,5,6,7,8}, P in "
+                        "{8,16,32,64}: 16..512 points, a power of two or 3, 5 or 7 times one)", n[d], d);
             *out = c;
             return -2;
         }

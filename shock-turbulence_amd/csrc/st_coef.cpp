@@ -263,7 +263,7 @@ double sponge_sigma(int nx, int i, double amp, double power, double frac)
 
 int choose_layout(int n, int *M, int *P)
 {
-    static const int cand[] = {8, 6, 4, 2};
+    static const int cand[] = {8, 7, 6, 5, 4, 2};
     for (int m : cand) {
         if (n % m) continue;
         const int p = n / m;

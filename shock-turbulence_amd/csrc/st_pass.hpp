@@ -223,7 +223,8 @@ static hipError_t launch_t(const Geo &g, const PassArgs &a, hipStream_t s)
 
 // The lane layouts (M points per lane, P lanes per line) that
 // choose_layout and its narrow variant (M/2, 2P) produce for lines of
-// 16..512 points (powers of two) and 48..384 (three times a power of two).
+// 16..512 points (powers of two), 48..384 (three times a power of two),
+// 40..320 (five times one) and 56..448 (seven times one).
 template <class PASS, int NT, bool NP>
 static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStream_t s)
 {
@@ -249,6 +250,15 @@ static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStr
         case 816: return launch_t<8, 16, NT, NP, PASS>(g, a, s);
         case 832: return launch_t<8, 32, NT, NP, PASS>(g, a, s);
         case 864: return launch_t<8, 64, NT, NP, PASS>(g, a, s);
+        // lines of 40, 80, 160, 320 (M = 5) and 56, 112, 224, 448 (M = 7) points
+        case 508: return launch_t<5, 8, NT, NP, PASS>(g, a, s);
+        case 516: return launch_t<5, 16, NT, NP, PASS>(g, a, s);
+        case 532: return launch_t<5, 32, NT, NP, PASS>(g, a, s);
+        case 564: return launch_t<5, 64, NT, NP, PASS>(g, a, s);
+        case 708: return launch_t<7, 8, NT, NP, PASS>(g, a, s);
+        case 716: return launch_t<7, 16, NT, NP, PASS>(g, a, s);
+        case 732: return launch_t<7, 32, NT, NP, PASS>(g, a, s);
+        case 764: return launch_t<7, 64, NT, NP, PASS>(g, a, s);
         // lines of 48, 96, 192 and 384 points
         case 316: return launch_t<3, 16, NT, NP, PASS>(g, a, s);
         case 332: return launch_t<3, 32, NT, NP, PASS>(g, a, s);

@@ -40,7 +40,7 @@ PER = (2.0, 2 * np.pi, 2 * np.pi)
 # x lines of 16..512 points: lanes per line P = 8, 16, 32, 64 and both
 # per-lane widths of each pass
 SHAPES = [(16, 32, 16), (32, 16, 16), (64, 16, 16), (128, 16, 32), (256, 16, 16), (512, 16, 16),
-          (48, 16, 16), (192, 16, 16)]
+          (48, 16, 16), (192, 16, 16), (80, 16, 16), (160, 16, 16), (448, 16, 16)]
 
 
 def _log(*a):

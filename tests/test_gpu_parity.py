@@ -30,11 +30,15 @@ def _log(*a):
 
 # Line lengths 16..256 cover every lane layout of the passes: lines of
 # P = 8, 16, 32 and 64 lanes (one to eight lines per wave).  Lines of 48,
-# 96, 192 and 384 points use 6 (or 3) points per lane.
+# 96, 192 and 384 points use 6 (or 3) points per lane, lines of 40..320
+# five and of 56..448 seven.
 SHAPES = [(32, 32, 32), (64, 32, 16), (16, 64, 32), (256, 16, 16), (16, 128, 32), (16, 16, 256),
           (48, 96, 16), (16, 48, 192), (384, 16, 16)]
 LONG = [(256, 16, 16), (16, 128, 32), (32, 16, 256)]
 THREE = [(48, 48, 48), (96, 16, 48), (16, 192, 16)]
+# lines of five and seven times a power of two (5 or 7 points per lane)
+FIVE_SEVEN = [(80, 16, 16), (16, 112, 16), (16, 16, 160), (40, 56, 16), (320, 16, 16), (16, 448, 16),
+              (16, 16, 224)]
 
 
 @pytest.fixture(scope="module")
@@ -52,7 +56,7 @@ def solvers():
         s.close()
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", SHAPES + FIVE_SEVEN)
 def test_operators(solvers, shape):
     _log("operators", shape)
     s, o = solvers(shape)
@@ -89,7 +93,7 @@ def test_d1_sinusoid(solvers, shape):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG + THREE)
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG + THREE + [(40, 56, 112)])
 def test_rhs(solvers, shape, hyper):
     _log("rhs", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
@@ -178,6 +182,22 @@ def test_steps_20_48(solvers):
     """The smallest served and batched grid (48^3: 6 points per lane)."""
     re = st_amd.re_for_taylor(40.0, 4.0, 1.0, 0.3)
     _run_steps(solvers, (48, 48, 48), 20, 1e-10, Re=re, Pr=0.7, hyper_param=1)
+
+
+def test_steps_20_80():
+    """80^3 (5 points per lane in every direction), 20 steps."""
+    re = st_amd.re_for_taylor(40.0, 4.0, 1.0, 0.3)
+    cache = {}
+
+    def get(shape, **kw):
+        if shape not in cache:
+            cache[shape] = (st_amd.Solver(*shape, **kw), O.Oracle(*shape, **kw))
+        return cache[shape]
+    try:
+        _run_steps(get, (80, 80, 80), 20, 1e-10, Re=re, Pr=0.7, hyper_param=1)
+    finally:
+        for s, _ in cache.values():
+            s.close()
 
 
 @pytest.mark.slow
