@@ -174,6 +174,15 @@ def ensemble_leg(nsteps, autotune):
     el = time.perf_counter() - t0
     ens.close()
     mix_rate = (work * (nsteps - k_alone) / nsteps) / el / 1e6
+    # the same instances as one group: one launch per pass over all four
+    # sizes (one step to capture the graph, then the same step count)
+    grp = st_serve.Ensemble(runs, grouped=True)
+    grp.march(1)
+    t0 = time.perf_counter()
+    grp.march(nsteps - k_alone)
+    el_g = time.perf_counter() - t0
+    grp.close()
+    group_rate = (work * (nsteps - k_alone) / nsteps) / el_g / 1e6
     plain_t = {}
     for r in runs:
         s = st_amd.Solver(r.n, r.n, r.n, Re=r.Re, Pr=0.7, hyper_param=1, cfl_cal=0.5)
@@ -198,8 +207,10 @@ def ensemble_leg(nsteps, autotune):
                         "Mt in {0.2,0.3,0.5}, Re_lambda=40; %d steps each" % nsteps,
             "points": sum(r.n ** 3 for r in runs), "steps": nsteps,
             "batched_mpts_stage_per_s": round(mix_rate, 1),
+            "grouped_mpts_stage_per_s": round(group_rate, 1),
             "plain_one_at_a_time_mpts_stage_per_s": round(plain_rate, 1),
-            "speedup": round(mix_rate / plain_rate, 3), "autotune_reps": autotune,
+            "speedup": round(mix_rate / plain_rate, 3),
+            "speedup_grouped": round(group_rate / plain_rate, 3), "autotune_reps": autotune,
             "per_size": per_size,
             # what the one-at-a-time rate per size allows: a speedup of x
             # over it needs the mix to run at x times the plain rate

@@ -178,6 +178,25 @@ int st_batch_status(st_ctx *ctx, int *step, double *time, double *dt_last, int *
  * it runs (does not wait). */
 int st_query(st_ctx *ctx);
 
+/* ---- groups of batches of different sizes (no counterpart in the
+ * reference).  For ensemble and serving callers with mixed grid sizes: a
+ * group steps up to four batches, one per size of the family 48^3, 64^3,
+ * 96^3, 128^3 (periodic, no sponge, one hyper_param and rk_scheme), with
+ * ONE launch per pass covering every member's running slots, instead of
+ * one launch per pass per batch.  Every pass of a member then takes that
+ * size's M/2-point layout (st_pass_config_set narrow = 1, nt = 256 gives a
+ * single batch the same layouts and so bit-identical results).  The
+ * members stay usable as batches (slot load, store, release, status act
+ * on them); st_group_march orders itself after the members' pending work
+ * and their later work after its steps.  A step is captured in a HIP graph
+ * once per tuple of active-slot counts.  Destroy the group before its
+ * members.  Errors: st_last_error(NULL). */
+typedef struct st_group st_group;
+int st_group_create(st_ctx *const *members, int nmembers, st_group **out);
+int st_group_march(st_group *group, int nsteps);
+int st_group_sync(st_group *group);
+void st_group_destroy(st_group *group);
+
 /* Autotuner (no counterpart in the reference): time every pass of an RK
  * stage over all slots under each lane layout (M or M/2 points per lane) and
  * tile width (256, 512, 1024 threads) and keep the fastest per pass and

@@ -6,7 +6,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../include/st_api.h"
@@ -29,6 +31,14 @@ const char *kind_names[NKIND] = {
 struct Event {
     hipEvent_t a, b;
     int kind;             // pass kind * 3 + direction
+};
+
+// A pass recorded instead of launched (the members of a group, see
+// group_step): its kind, direction, prepared arguments and host tables.
+struct Rec {
+    int kind, dir;
+    st::PassArgs a;
+    const st::DirTab *t;
 };
 
 }  // namespace
@@ -70,6 +80,8 @@ struct st_ctx {
     double prof_ms[NKIND * 3];      // per pass kind and direction
     long prof_cnt[NKIND * 3];
     double prof_fields[NKIND * 3];
+    std::vector<Rec> *rec = nullptr;  // set: pass() records instead of launching
+    int force_narrow = 0;             // 1: every pass takes the M/2-point layout
 };
 
 namespace {
@@ -196,11 +208,15 @@ double kind_fields(int kind, int dir, const st::PassArgs &a)
 int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
 {
     const int pk = kind * 3 + dir;
-    const int lay = c->cfg_narrow[kind][dir] ? 3 : 0;
+    const int lay = c->force_narrow || c->cfg_narrow[kind][dir] ? 3 : 0;
     const st::DirTab &t = c->th[lay + dir];
     a.tab = c->td + lay + dir;
     a.K1 = cadd * c->sx1[dir] * (double)(t.n - 1) * t.op[0].scale;   // fac of src/derivCDopt6_periodic.f:181
     a.K2 = t.s2 * t.op[1].scale;
+    if (c->rec) {
+        c->rec->push_back(Rec{kind, dir, a, &t});
+        return 0;
+    }
     Event e;
     const int prof = c->prof && !a.dry;
     if (prof) prof_begin(c, pk, e);
@@ -1139,6 +1155,243 @@ extern "C" int st_batch_status(st_ctx *c, int *step, double *time, double *dt_la
 }
 
 extern "C" int st_batch_slots(const st_ctx *c) { return c ? c->ns : 0; }
+
+// ---------------------------------------------------------------------
+// groups: batches of different grid sizes stepped together, one ragged
+// launch per pass (no reference counterpart)
+// ---------------------------------------------------------------------
+struct st_group {
+    st_ctx *m[4] = {nullptr, nullptr, nullptr, nullptr};   // by size-family member
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;                                 // orders the member streams with s
+    std::map<std::array<int, 4>, hipGraphExec_t> graphs;     // one step per tuple of active-slot counts
+};
+
+namespace {
+
+// Record one stage of member m (nothing is launched); the member's passes
+// all take the M/2-point layouts of the size family.
+int group_record(st_ctx *m, const st::PassArgs &a, double ra, double rb, std::vector<Rec> &out)
+{
+    out.clear();
+    m->rec = &out;
+    m->force_narrow = 1;
+    const int r = seq_stage(m, a, ra, rb, 0.0);
+    m->rec = nullptr;
+    m->force_narrow = 0;
+    return r;
+}
+
+// One step of every member's running slots, on the group's stream: per
+// member the device dt (as batch_step), then the stages with each pass
+// one ragged launch over all members.
+int group_step(st_group *G)
+{
+    st::PassArgs a[4];
+    int c0 = -1;
+    hipError_t err = hipSuccess;
+    for (int c = 0; c < 4; c++) {
+        st_ctx *m = G->m[c];
+        if (!m || !m->nact) continue;
+        if (c0 < 0) c0 = c;
+        st::PassArgs &b = a[c];
+        b = base_args(m);
+        b.nslot = m->nact;
+        b.slots = m->s_list;
+        b.act = m->s_act;
+        int nb = 0;
+        double *pc = m->red, *pt = m->red + (long)st::reduce_blocks(m->ni) * m->ns * 2;
+        err = st::run_reduce(0, b, pc, &nb, G->s);
+        if (err == hipSuccess && m->p.hyper_param == 1) err = st::run_reduce(1, b, pt, &nb, G->s);
+        if (err == hipSuccess)
+            err = st::run_dt(pc, pt, nb, m->ns, m->s_act, m->s_target, m->s_step, m->p.cfl_cal, m->p.hyper_param, 0,
+                             m->s_run, m->s_dt, m->s_dtlast, G->s);
+        if (err != hipSuccess) return st::set_call_error(-4, "group dt (%d points): %s", m->Mx, hipGetErrorString(err));
+        b.act = m->s_run;
+        b.sre = m->s_re;
+        b.sdt = m->s_dt;
+    }
+    if (c0 < 0) return 0;
+    const st_params &p = G->m[c0]->p;
+    std::vector<Rec> rec[4];
+    for (int loop = 0; loop < 2; loop++) {
+        const double *ra, *rb;
+        const int ns = rk_loop(p.rk_scheme, loop, &ra, &rb);
+        if (!ns) break;
+        for (int c = c0; c < 4; c++) {
+            st_ctx *m = G->m[c];
+            if (!m || !m->nact) continue;
+            err = st::run_fill_slots(m->QW, 5, m->N, m->ni, a[c].nslot, a[c].slots, a[c].act, 0.0, G->s);
+            if (err != hipSuccess) return st::set_call_error(-4, "group fill: %s", hipGetErrorString(err));
+        }
+        for (int k = 0; k < ns; k++) {
+            for (int c = c0; c < 4; c++) {
+                st_ctx *m = G->m[c];
+                if (!m || !m->nact) continue;
+                if (group_record(m, a[c], ra[k], rb[k], rec[c]))
+                    return st::set_call_error(-4, "group stage (%d points): %s", m->Mx, m->err);
+                if (rec[c].size() != rec[c0].size())
+                    return st::set_call_error(-4, "group stage: members record different pass sequences");
+            }
+            for (size_t i = 0; i < rec[c0].size(); i++) {
+                st::RaggedSpec sp;
+                std::memset(&sp, 0, sizeof(sp));
+                const Rec &r0 = rec[c0][i];
+                sp.dir = r0.dir;
+                sp.first = c0;
+                for (int c = c0; c < 4; c++) {
+                    st_ctx *m = G->m[c];
+                    if (!m || !m->nact) continue;
+                    const Rec &r = rec[c][i];
+                    if (r.kind != r0.kind || r.dir != r0.dir)
+                        return st::set_call_error(-4, "group stage: members record different pass sequences");
+                    sp.t[c] = r.t;
+                    sp.My[c] = m->My;
+                    sp.Mz[c] = m->Mz;
+                    sp.Mx[c] = m->Mx;
+                    sp.a[c] = r.a;
+                }
+                err = st::run_pass_ragged(r0.kind, sp, G->s);
+                if (err != hipSuccess)
+                    return st::set_call_error(-4, "group pass %s dir %d: %s", kind_names[r0.kind], r0.dir,
+                                              hipGetErrorString(err));
+            }
+        }
+        const int last = loop == 1 || !rk_loop(p.rk_scheme, 1, &ra, &rb);
+        for (int c = c0; c < 4; c++) {
+            st_ctx *m = G->m[c];
+            if (!m || !m->nact) continue;
+            err = st::run_clock(m->ns, m->s_run, m->s_dt, m->s_time, m->s_step, last, G->s);
+            if (err != hipSuccess) return st::set_call_error(-4, "group clock: %s", hipGetErrorString(err));
+        }
+    }
+    return 0;
+}
+
+#define GCHK(x)                                                                                    \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return st::set_call_error(-3, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+// Capture one group step per tuple of active-slot counts.
+int group_graph(st_group *G, const std::array<int, 4> &key)
+{
+    if (G->graphs.count(key)) return 0;
+    hipGraph_t g = nullptr;
+    GCHK(hipStreamBeginCapture(G->s, hipStreamCaptureModeRelaxed));
+    const int r = group_step(G);
+    const hipError_t e = hipStreamEndCapture(G->s, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return st::set_call_error(-3, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return st::set_call_error(-3, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    G->graphs[key] = x;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int st_group_create(st_ctx *const *members, int nmembers, st_group **out)
+{
+    if (!members || !out || nmembers < 1 || nmembers > 4)
+        return st::set_call_error(-1, "st_group_create: 1 to 4 member batches");
+    *out = nullptr;
+    st_group *G = new st_group();
+    for (int i = 0; i < nmembers; i++) {
+        st_ctx *m = members[i];
+        if (!m) {
+            delete G;
+            return st::set_call_error(-1, "st_group_create: member %d is null", i);
+        }
+        const int c = st::ragged_member(m->Mx);
+        if (c < 0 || m->My != m->Mx || m->Mz != m->Mx) {
+            delete G;
+            return st::set_call_error(-1, "st_group_create: a member grid must be 48^3, 64^3, 96^3 or 128^3 (got %dx%dx%d)",
+                                      m->Mx, m->My, m->Mz);
+        }
+        if (G->m[c]) {
+            delete G;
+            return st::set_call_error(-1, "st_group_create: two members of %d^3 (one batch per size)", m->Mx);
+        }
+        if (m->g.nonperiodic_x || m->spg) {
+            delete G;
+            return st::set_call_error(-1, "st_group_create: members must be periodic, without a sponge");
+        }
+        const st_ctx *m0 = members[0];
+        if (m->p.hyper_param != m0->p.hyper_param || m->p.rk_scheme != m0->p.rk_scheme) {
+            delete G;
+            return st::set_call_error(-1, "st_group_create: members must share hyper_param and rk_scheme");
+        }
+        for (int d = 0; d < 3; d++)
+            if (m->th[3 + d].M != st::RAGGED_M[c] || m->th[3 + d].P != st::RAGGED_P[c]) {
+                delete G;
+                return st::set_call_error(-1, "st_group_create: %d^3 has no %d-point layout on %d lanes", m->Mx,
+                                          st::RAGGED_M[c], st::RAGGED_P[c]);
+            }
+        G->m[c] = m;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&G->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (G->s) (void)hipStreamDestroy(G->s);
+        delete G;
+        return st::set_call_error(-3, "st_group_create: %s", hipGetErrorString(e));
+    }
+    *out = G;
+    return 0;
+}
+
+extern "C" int st_group_march(st_group *G, int nsteps)
+{
+    if (!G || nsteps < 0) return st::set_call_error(-1, "st_group_march: bad arguments");
+    std::array<int, 4> key{0, 0, 0, 0};
+    int any = 0;
+    // the group's stream waits for the members' own work (loads, releases)
+    for (int c = 0; c < 4; c++) {
+        st_ctx *m = G->m[c];
+        if (!m) continue;
+        key[c] = m->nact;
+        any |= m->nact;
+        GCHK(hipEventRecord(G->ev, m->s));
+        GCHK(hipStreamWaitEvent(G->s, G->ev, 0));
+    }
+    if (!any || !nsteps) return 0;
+    if (getenv("ST_NO_GRAPH")) {
+        for (int i = 0; i < nsteps; i++) TRY(group_step(G));
+    } else {
+        TRY(group_graph(G, key));
+        const hipGraphExec_t x = G->graphs[key];
+        for (int i = 0; i < nsteps; i++) GCHK(hipGraphLaunch(x, G->s));
+    }
+    // and the members' later work waits for the steps
+    GCHK(hipEventRecord(G->ev, G->s));
+    for (int c = 0; c < 4; c++)
+        if (G->m[c]) GCHK(hipStreamWaitEvent(G->m[c]->s, G->ev, 0));
+    return 0;
+}
+
+extern "C" int st_group_sync(st_group *G)
+{
+    if (!G) return st::set_call_error(-1, "st_group_sync: null group");
+    GCHK(hipStreamSynchronize(G->s));
+    return 0;
+}
+
+extern "C" void st_group_destroy(st_group *G)
+{
+    if (!G) return;
+    (void)hipStreamSynchronize(G->s);
+    for (auto &kv : G->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (G->ev) (void)hipEventDestroy(G->ev);
+    if (G->s) (void)hipStreamDestroy(G->s);
+    delete G;
+}
 
 extern "C" int st_query(st_ctx *c)
 {

@@ -37,9 +37,8 @@ struct Geo {
 // Tile for this workgroup.  Workgroups are dispatched round-robin over the
 // 8 XCDs; consecutive tiles (which share 128-byte lines for the strided
 // directions) are handed to workgroups of the same XCD.
-DEV long tile_of(const Geo &g)
+DEV long tile_of(const Geo &g, long wg)
 {
-    const long wg = blockIdx.x;
     if ((g.ntiles & 7) == 0) return (wg & 7) * (g.ntiles >> 3) + (wg >> 3);
     return wg;
 }

@@ -66,6 +66,33 @@ enum PassKind {
     K_NPASS,              // pass kinds end here
 };
 
+// One ragged launch: a pass over a group of batches of different grid sizes
+// (cubic grids of 48, 64, 96 or 128 points, member c of the size family,
+// see ragged_member).  Per member: its host tables (null: absent), grid and
+// pass arguments; `first` is the member whose arguments pick the pass
+// variant (all members run the same stage sequence).
+struct RaggedSpec {
+    int dir;
+    int first;
+    const DirTab *t[4];
+    int My[4], Mz[4], Mx[4];
+    PassArgs a[4];
+};
+// The size family of ragged launches: lines of RAGGED_N[c] points in the
+// layout of RAGGED_M[c] points per lane on RAGGED_P[c] lanes (each size's
+// narrow layout).
+constexpr int RAGGED_N[4] = {48, 64, 96, 128};
+constexpr int RAGGED_M[4] = {3, 4, 3, 4};
+constexpr int RAGGED_P[4] = {16, 16, 32, 32};
+// Family member of a line length (-1: not in the family).
+int ragged_member(int n);
+hipError_t run_pass_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+hipError_t launch_grad_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+hipError_t launch_lapse_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+hipError_t launch_hyper_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+hipError_t launch_hdivrk_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+hipError_t launch_filter_ragged(int kind, const RaggedSpec &sp, hipStream_t s);
+
 // Default launch layout of a pass kind: narrow = 1 uses the tables with half
 // the points per lane (twice the lanes per line); nt = threads per workgroup.
 void pass_default_cfg(int kind, int dir, int *narrow, int *nt);

@@ -61,6 +61,30 @@ hipError_t run_pass(int kind, const DirTab &t, int nt, int dir, int My, int Mz, 
     }
 }
 
+int ragged_member(int n)
+{
+    for (int c = 0; c < 4; c++)
+        if (RAGGED_N[c] == n) return c;
+    return -1;
+}
+
+hipError_t run_pass_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    switch (kind) {
+    case K_GRAD: case K_GRAD_CROSS:
+        return launch_grad_ragged(kind, sp, s);
+    case K_LAPSE:
+        return launch_lapse_ragged(kind, sp, s);
+    case K_LAP2: case K_LAP2_LAST: case K_GAUSS_ETA: case K_GAUSS_ETA_LAST: case K_HDIV: case K_LAPSE_LAP2:
+        return launch_hyper_ragged(kind, sp, s);
+    case K_RKF: case K_FQ: case K_FQ_FIX:
+        return launch_filter_ragged(kind, sp, s);
+    case K_HDIV_RKF:
+        return launch_hdivrk_ragged(kind, sp, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace st
 
 // ---------------------------------------------------------------------

@@ -58,8 +58,7 @@ struct has_mid<T, std::void_t<decltype(T::MID)>> { static constexpr bool value =
 // store phases (same points per thread): store_early() fills it, store()
 // reads it, so early results can feed the late store without a slot.
 template <int M, int P, int NT, bool NP, class PASS>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
-k_pass(Geo g, PassArgs a0)
+DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
 {
     extern __shared__ double lds[];
     constexpr int NS = PASS::NSLOT, NO = PASS::NOUT, NE = PASS::NEARLY;
@@ -73,7 +72,6 @@ k_pass(Geo g, PassArgs a0)
     constexpr int NCR = carry_n<PASS>::value;
     static_assert(NCR == 0 || (RE && NO > 0), "a carry goes from store_early to store");
     double cr[M][NCR > 0 ? NCR : 1];
-    const long tile = tile_of(g);
     const int si = (int)(tile / g.tpi);
     const int slot = a0.slots ? a0.slots[si] : si;
     if (a0.act && !a0.act[slot]) return;          // uniform per workgroup, before any barrier
@@ -168,6 +166,13 @@ k_pass(Geo g, PassArgs a0)
         if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
         else PASS::store(g, a, gi, v);
     }
+}
+
+template <int M, int P, int NT, bool NP, class PASS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
+k_pass(Geo g, PassArgs a0)
+{
+    pass_body<M, P, NT, NP, PASS>(g, a0, tile_of(g, blockIdx.x));
 }
 
 // Passes whose early outputs have their own slots.
@@ -306,5 +311,111 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
     return hipErrorInvalidValue;
 }
 
+
+// ---------------------------------------------------------------------
+// Ragged launches: one launch of a pass over the instances of several
+// grid sizes, each size a batch of its own (a group).  The sizes belong to
+// a family fixed at compile time, each with its own lane layout; class c
+// (the family member) owns workgroups [start[c], start[c+1]), padded to a
+// multiple of 8 so that a class's consecutive tiles keep to one XCD.
+// ---------------------------------------------------------------------
+constexpr int RG_MAX = 4;
+template <int M_, int P_>
+struct Lay {
+    static constexpr int M = M_, P = P_;
+};
+// The ensemble and serving family (RAGGED_N, RAGGED_M, RAGGED_P).
+struct FamilyServe {
+    using L0 = Lay<RAGGED_M[0], RAGGED_P[0]>;
+    using L1 = Lay<RAGGED_M[1], RAGGED_P[1]>;
+    using L2 = Lay<RAGGED_M[2], RAGGED_P[2]>;
+    using L3 = Lay<RAGGED_M[3], RAGGED_P[3]>;
+    static int M_of(int c) { return RAGGED_M[c]; }
+    static int P_of(int c) { return RAGGED_P[c]; }
+};
+
+struct RaggedArgs {
+    Geo g[RG_MAX];
+    PassArgs a[RG_MAX];
+    long start[RG_MAX + 1];
+};
+static_assert(sizeof(RaggedArgs) <= 4000, "kernel arguments");
+
+template <int NT, class PASS, class F>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PASS::MIN_WAVES)))
+k_ragged(RaggedArgs r)
+{
+    const long wg = blockIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int i = 1; i < RG_MAX; i++) c += wg >= r.start[i] ? 1 : 0;
+    const long local = wg - r.start[c];
+    switch (c) {
+    case 0:
+        if (local < r.g[0].ntiles)
+            pass_body<F::L0::M, F::L0::P, NT, false, PASS>(r.g[0], r.a[0], tile_of(r.g[0], local));
+        break;
+    case 1:
+        if (local < r.g[1].ntiles)
+            pass_body<F::L1::M, F::L1::P, NT, false, PASS>(r.g[1], r.a[1], tile_of(r.g[1], local));
+        break;
+    case 2:
+        if (local < r.g[2].ntiles)
+            pass_body<F::L2::M, F::L2::P, NT, false, PASS>(r.g[2], r.a[2], tile_of(r.g[2], local));
+        break;
+    default:
+        if (local < r.g[3].ntiles)
+            pass_body<F::L3::M, F::L3::P, NT, false, PASS>(r.g[3], r.a[3], tile_of(r.g[3], local));
+        break;
+    }
+}
+
+// One ragged launch of PASS: per family member c, the tables t[c] (null:
+// no instances of that size), the grid and the pass arguments.  Every
+// member present must have its family layout and the periodic tables.
+template <class PASS, class F = FamilyServe>
+static hipError_t launch_ragged(const RaggedSpec &sp, hipStream_t s)
+{
+    constexpr int NT = 256;
+    RaggedArgs r;
+    long wg = 0;
+    size_t lds = 0;
+    for (int c = 0; c < RG_MAX; c++) {
+        r.start[c] = wg;
+        Geo &g = r.g[c];
+        g = Geo();
+        r.a[c] = sp.a[c];
+        g.ntiles = 0;
+        if (!sp.t[c] || sp.a[c].nslot < 1) continue;
+        const DirTab &t = *sp.t[c];
+        if (t.M != F::M_of(c) || t.P != F::P_of(c) || t.nonper) return hipErrorInvalidValue;
+        g.dir = sp.dir;
+        g.n = t.n;
+        g.My = sp.My[c];
+        g.Mz = sp.Mz[c];
+        g.Mx = sp.Mx[c];
+        g.P = t.P;
+        g.B = NT / t.P;
+        const long lot = (long)g.My * g.Mz * g.Mx / g.n;
+        if (lot % g.B || sp.a[c].ni != (long)g.My * g.Mz * g.Mx) return hipErrorInvalidValue;
+        g.tpi = lot / g.B;
+        g.ntiles = g.tpi * sp.a[c].nslot;
+        wg += (g.ntiles + 7) / 8 * 8;
+        const size_t l = (size_t)pass_slots<PASS>() * NT * t.M * sizeof(double);
+        lds = l > lds ? l : lds;
+    }
+    r.start[RG_MAX] = wg;
+    if (wg == 0) return hipSuccess;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static size_t lds_set = 0;
+    if (lds > 64 * 1024 && lds > lds_set) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ragged<NT, PASS, F>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL((k_ragged<NT, PASS, F>), dim3((unsigned)wg), dim3(NT), lds, s, r);
+    return hipGetLastError();
+}
 
 }  // namespace st

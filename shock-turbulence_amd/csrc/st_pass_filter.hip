@@ -83,4 +83,14 @@ hipError_t launch_filter(int kind, const DirTab &t, int nt, int dir, int My, int
     }
 }
 
+hipError_t launch_filter_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    switch (kind) {
+    case K_RKF: return launch_ragged<PassFilterQ<0>>(sp, s);
+    case K_FQ: return launch_ragged<PassFilterQ<1>>(sp, s);
+    case K_FQ_FIX: return launch_ragged<PassFilterQ<2>>(sp, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace st

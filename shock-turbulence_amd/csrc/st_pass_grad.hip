@@ -391,4 +391,13 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
     return hipErrorInvalidValue;
 }
 
+// The 256-thread variants (those of the plain path below 1024 threads).
+hipError_t launch_grad_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    if (kind == K_GRAD && sp.dir == 0) return launch_ragged<PassGrad<1, false, false, 0>>(sp, s);
+    if (kind == K_GRAD && sp.dir == 2) return launch_ragged<PassGrad<1, false, false, 2>>(sp, s);
+    if (kind == K_GRAD_CROSS && sp.dir == 1) return launch_ragged<PassGrad<2, false, true, 4>>(sp, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace st

@@ -96,4 +96,10 @@ hipError_t launch_hdivrk(int kind, const DirTab &t, int nt, int dir, int My, int
     return launch<PassHdivRkf>(t, nt, dir, My, Mz, Mx, a, s);
 }
 
+hipError_t launch_hdivrk_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    if (kind != K_HDIV_RKF || sp.dir == 2) return hipErrorInvalidValue;
+    return launch_ragged<PassHdivRkf>(sp, s);
+}
+
 }  // namespace st

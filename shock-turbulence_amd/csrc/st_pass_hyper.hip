@@ -249,4 +249,17 @@ hipError_t launch_hyper(int kind, const DirTab &t, int nt, int dir, int My, int 
     }
 }
 
+hipError_t launch_hyper_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    switch (kind) {
+    case K_LAP2: return launch_ragged<PassLap2>(sp, s);
+    case K_LAP2_LAST: return launch_ragged<PassLap2Gauss>(sp, s);
+    case K_LAPSE_LAP2: return launch_ragged<PassLapSELap2>(sp, s);
+    case K_GAUSS_ETA: return launch_ragged<PassGaussEta>(sp, s);
+    case K_GAUSS_ETA_LAST: return launch_ragged<PassGaussHdiv>(sp, s);
+    case K_HDIV: return launch_ragged<PassHyperDiv>(sp, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace st

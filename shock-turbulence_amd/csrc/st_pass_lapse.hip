@@ -65,4 +65,11 @@ hipError_t launch_lapse(int kind, const DirTab &t, int nt, int dir, int My, int 
     return launch<PassLapSE<true>>(t, nt, dir, My, Mz, Mx, a, s);
 }
 
+hipError_t launch_lapse_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
+{
+    if (kind != K_LAPSE) return hipErrorInvalidValue;
+    if (sp.a[sp.first].first) return launch_ragged<PassLapSE<false>>(sp, s);
+    return launch_ragged<PassLapSE<true>>(sp, s);
+}
+
 }  // namespace st

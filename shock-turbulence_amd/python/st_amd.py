@@ -62,7 +62,8 @@ EXPORTS = [
     "st_ic_isotropic", "st_batch_create", "st_batch_slots", "st_slot_load", "st_slot_store", "st_slot_release",
     "st_batch_march", "st_batch_status", "st_autotune", "st_pass_config", "st_pass_config_set", "st_query",
     "st_restart_write", "st_restart_read", "st_deck_read", "st_sponge", "st_ic_shock", "st_compute_hyper",
-    "st_compute_grad", "st_cfl_noalloc", "st_build_id",
+    "st_compute_grad", "st_cfl_noalloc", "st_build_id", "st_group_create", "st_group_march", "st_group_sync",
+    "st_group_destroy",
 ]
 
 _lib = None
@@ -135,6 +136,11 @@ def lib():
                               ctypes.c_ulonglong, _dp, _dp]
     L.st_pass_config.argtypes = [vp, ctypes.c_int, ip, ip]
     L.st_pass_config_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.st_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    L.st_group_march.argtypes = [vp, ctypes.c_int]
+    L.st_group_sync.argtypes = [vp]
+    L.st_group_destroy.argtypes = [vp]
+    L.st_group_destroy.restype = None
     _lib = L
     return L
 
@@ -498,3 +504,49 @@ class Batch(Solver):
         dt = np.zeros(n)
         self._chk(lib().st_batch_status(self.h, step, _p(t), _p(dt), act), "st_batch_status")
         return np.array(step[:]), t, dt, np.array(act[:])
+
+
+# the grid sizes a Group takes (cubic, one batch each)
+GROUP_SIZES = (48, 64, 96, 128)
+
+
+def narrow_configs(solver):
+    """Every pass in its M/2-point layout on 256-thread tiles: the layouts a
+    Group uses, so a Batch with these configs gives the same bits."""
+    return {k: (1, 256) for k in solver.pass_configs()}
+
+
+class Group:
+    """Batches of different sizes (one per size in GROUP_SIZES) stepped
+    together, one launch per pass over all of them (st_group_*).  The
+    batches stay usable for load/store/release/status; close the group
+    before them."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        arr = (ctypes.c_void_p * len(self.batches))(*[b.h.value for b in self.batches])
+        h = ctypes.c_void_p()
+        r = lib().st_group_create(arr, len(self.batches), ctypes.byref(h))
+        if r:
+            raise StError("st_group_create: %s (%d)" % (lib().st_last_error(None).decode(), r))
+        self.h = h
+
+    def _chk(self, r, what):
+        if r < 0:
+            raise StError("%s: %s (%d)" % (what, lib().st_last_error(None).decode(), r))
+        return r
+
+    def march(self, nsteps):
+        """Enqueue nsteps group steps (asynchronous; sync() or a member's status() waits)."""
+        self._chk(lib().st_group_march(self.h, nsteps), "st_group_march")
+
+    def sync(self):
+        self._chk(lib().st_group_sync(self.h), "st_group_sync")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().st_group_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()

@@ -105,21 +105,35 @@ def pool_sizes(runs, budget_bytes):
 
 
 class Ensemble:
-    """Advance a fixed set of runs together: one pool per size."""
+    """Advance a fixed set of runs together: one pool per size.  With
+    grouped (sizes of st_amd.GROUP_SIZES only), the pools step as one
+    st_amd.Group, one launch per pass over all sizes, in the layouts of
+    st_amd.narrow_configs; otherwise each pool steps on its own stream."""
 
-    def __init__(self, runs, autotune=0, **kw):
+    def __init__(self, runs, autotune=0, grouped=False, **kw):
         self.runs = runs
         self.pools = {}
+        self.group = None
         for n, s in sorted(collections.Counter(r.n for r in runs).items()):
             self.pools[n] = Pool(n, s, _params(kw))
         for r in runs:
             self.pools[r.n].admit(r, 0.0)
         self.tuned = {}
-        if autotune:
+        if grouped:
+            if autotune:
+                raise ValueError("a group takes fixed layouts: no autotune")
+            self.group = st_amd.Group([p.batch for p in self.pools.values()])
+            for n, p in self.pools.items():
+                self.tuned[n] = st_amd.narrow_configs(p.batch)
+        elif autotune:
             for n, p in self.pools.items():
                 self.tuned[n] = p.batch.autotune(autotune)
 
     def march(self, nsteps):
+        if self.group is not None:
+            self.group.march(nsteps)
+            self.group.sync()
+            return
         for p in self.pools.values():
             p.batch.march(nsteps)
         for p in self.pools.values():
@@ -133,6 +147,9 @@ class Ensemble:
         return out
 
     def close(self):
+        if self.group is not None:
+            self.group.close()
+            self.group = None
         for p in self.pools.values():
             p.close()
 
