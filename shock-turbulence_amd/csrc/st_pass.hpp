@@ -37,6 +37,22 @@ struct keep_n { static constexpr int value = 0; };
 template <class T>
 struct keep_n<T, std::void_t<decltype(T::NKEEP)>> { static constexpr int value = T::NKEEP; };
 
+// NSF / NSFE of a pass (0 when it names none): global values per point
+// that its late (store) or early (store_early) store phase reads.  pass_body
+// loads them with sfetch() / sfetch_early() for SFB points (SF_BATCH, default
+// all M) before the first store of the batch, so that the loads are in
+// flight together instead of each waiting behind the stores before it (the
+// compiler may not move a load above a store to a pointer that could alias
+// it).  store() / store_early() then take them as a last argument.
+template <class T, class = void>
+struct sf_n { static constexpr int value = 0; static constexpr int batch = 0; };
+template <class T>
+struct sf_n<T, std::void_t<decltype(T::NSF)>> { static constexpr int value = T::NSF; static constexpr int batch = T::SF_BATCH; };
+template <class T, class = void>
+struct sfe_n { static constexpr int value = 0; };
+template <class T>
+struct sfe_n<T, std::void_t<decltype(T::NSFE)>> { static constexpr int value = T::NSFE; };
+
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
 template <class T>
@@ -119,8 +135,19 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     }
 
     // 3. store (through LDS, memory order)
+    constexpr int NSFE = sfe_n<PASS>::value, NSF = sf_n<PASS>::value;
+    static_assert((NSF == 0 && NSFE == 0) || (RE && NCR > 0 && NPF == 0),
+                  "store-phase fetches: passes with a carry, without prefetch");
     __syncthreads();
     if (RE) {
+        double sfe[M][NSFE > 0 ? NSFE : 1];
+        if constexpr (NSFE > 0) {
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                int b, s;
+                PASS::sfetch_early(g, a, W.at(g, r, b, s), sfe[r]);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < M; r++) {
             int b, s;
@@ -131,7 +158,8 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
 #pragma unroll
             for (int j = 0; j < NPF; j++) v[NE + j] = pf[r][j];
-            if constexpr (NCR > 0) PASS::store_early(g, a, gi, v, cr[r]);
+            if constexpr (NSFE > 0) PASS::store_early(g, a, gi, v, cr[r], sfe[r]);
+            else if constexpr (NCR > 0) PASS::store_early(g, a, gi, v, cr[r]);
             else PASS::store_early(g, a, gi, v);
         }
         if constexpr (NO == 0) return;
@@ -143,28 +171,45 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
         for (int k = 0; k < M; k++) L.st(j, k, out[j][k]);
     }
     __syncthreads();
+    constexpr int SFB = NSF > 0 && sf_n<PASS>::batch < M ? sf_n<PASS>::batch : M;
+    static_assert(SFB > 0, "store-fetch batch");
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        int b, s;
-        const long gi = W.at(g, r, b, s);
-        const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
-        constexpr int NV = RE ? NO : NO + NE;
-        constexpr int NK = keep_n<PASS>::value;
-        static_assert(NK == 0 || (RE && NS - NK >= NO && NS - NK >= NE), "kept slots hold no results");
-        double v[NV + NK + NPF > 0 ? NV + NK + NPF : 1];
+    for (int r0 = 0; r0 < M; r0 += SFB) {
+        double sf[SFB][NSF > 0 ? NSF : 1];
+        if constexpr (NSF > 0) {
 #pragma unroll
-        for (int j = 0; j < NO; j++) v[j] = lds[j * ss + li];
-        if (!RE) {
-#pragma unroll
-            for (int j = NO; j < NV; j++) v[j] = lds[(NS + j - NO) * ss + li];
+            for (int j = 0; j < SFB; j++) {
+                if (r0 + j >= M) break;
+                int b, s;
+                PASS::sfetch(g, a, W.at(g, r0 + j, b, s), sf[j]);
+            }
         }
 #pragma unroll
-        for (int j = 0; j < NK; j++) v[NV + j] = lds[(NS - NK + j) * ss + li];
+        for (int j = 0; j < SFB; j++) {
+            if (r0 + j >= M) break;
+            const int r = r0 + j;
+            int b, s;
+            const long gi = W.at(g, r, b, s);
+            const int k = s % M, c = s / M;
+            const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
+            constexpr int NV = RE ? NO : NO + NE;
+            constexpr int NK = keep_n<PASS>::value;
+            static_assert(NK == 0 || (RE && NS - NK >= NO && NS - NK >= NE), "kept slots hold no results");
+            double v[NV + NK + NPF > 0 ? NV + NK + NPF : 1];
 #pragma unroll
-        for (int j = 0; j < NPF; j++) v[NV + NK + j] = pf[r][j];
-        if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
-        else PASS::store(g, a, gi, v);
+            for (int q = 0; q < NO; q++) v[q] = lds[q * ss + li];
+            if (!RE) {
+#pragma unroll
+                for (int q = NO; q < NV; q++) v[q] = lds[(NS + q - NO) * ss + li];
+            }
+#pragma unroll
+            for (int q = 0; q < NK; q++) v[NV + q] = lds[(NS - NK + q) * ss + li];
+#pragma unroll
+            for (int q = 0; q < NPF; q++) v[NV + NK + q] = pf[r][q];
+            if constexpr (NSF > 0) PASS::store(g, a, gi, v, cr[r], sf[j]);
+            else if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
+            else PASS::store(g, a, gi, v);
+        }
     }
 }
 

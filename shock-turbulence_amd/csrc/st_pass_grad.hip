@@ -49,6 +49,11 @@ struct PassGrad {
     static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
+    // CROSS: the store phases' global reads (sfetch_early: the z and x
+    // partials of the Laplacian of e; sfetch: the state, four x and z
+    // derivatives, and the z and x partial sums) go out for all the
+    // thread's points (two at a time) before the first store
+    static constexpr int NSFE = CROSS ? 2 : 0, NSF = CROSS ? 19 : 0, SF_BATCH = 2;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     // G index of mixed input m: y: gxx=0, gyx=1, gyz=7, gzz=8; z: gxx=0, gzx=2
     HD static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
@@ -313,7 +318,13 @@ struct PassGrad {
 
     // CROSS: G(., y) and dT/dy go out and stay in the carry; the Laplacian
     // of e adds the z, then the x partial.
-    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v, double *cr)
+    DEV static void sfetch_early(const Geo &, const PassArgs &a, long gi, double *sf)
+    {
+        sf[0] = a.L2e ? a.L2ep[gi] : 0.0;
+        sf[1] = a.L2e ? a.L2ep[a.N + gi] : 0.0;
+    }
+    DEV static void store_early(const Geo &, const PassArgs &a, long gi, const double *v, double *cr,
+                                const double *sf)
     {
         const long N = a.N;
 #pragma unroll
@@ -321,7 +332,21 @@ struct PassGrad {
         a.G[10 * N + gi] = v[3];
 #pragma unroll
         for (int c = 0; c < 4; c++) cr[c] = v[c];
-        if (a.L2e) a.L2e[gi] = (v[4] + a.L2ep[gi]) + a.L2ep[N + gi];
+        if (a.L2e) a.L2e[gi] = (v[4] + sf[0]) + sf[1];
+    }
+    // the store phase's reads: Q (5), du/dz, dw/dx, dT/dx, dT/dz, the z
+    // partials (5), the x partials (5)
+    DEV static void sfetch(const Geo &, const PassArgs &a, long gi, double *sf)
+    {
+        const long N = a.N;
+#pragma unroll
+        for (int c = 0; c < 5; c++) sf[c] = a.Q[c * N + gi];
+        sf[5] = a.G[2 * N + gi];
+        sf[6] = a.G[6 * N + gi];
+        sf[7] = a.G[9 * N + gi];
+        sf[8] = a.G[11 * N + gi];
+#pragma unroll
+        for (int c = 0; c < 10; c++) sf[9 + c] = a.dQp[c * N + gi];
     }
     // CROSS: viscous terms built from products of first derivatives
     // (src/rhsv_periodic_sf.f:1128-1154 without the mixed and own-direction
@@ -330,31 +355,31 @@ struct PassGrad {
     // (src/hyper_dynamic.f:142-160; skipped when Cb is null: the
     // constant-coefficient src/hyper_routines.f); |S| and C_beta only with
     // the artificial transport on (L2e set).
-    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v, const double *cr)
+    DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v, const double *cr,
+                          const double *sf)
     {
         const long N = a.N;
         const Phys &ph = a.ph;
-        const Prim pr = prim(ph, a.Q[gi], a.Q[N + gi], a.Q[2 * N + gi], a.Q[3 * N + gi], a.Q[4 * N + gi]);
-        const double *G = a.G;
+        const Prim pr = prim(ph, sf[0], sf[1], sf[2], sf[3], sf[4]);
         static_assert(!CROSS || (NMX == 4 && msrc(0) == 0 && msrc(1) == 1 && msrc(2) == 7 && msrc(3) == 8),
                       "the kept slots are du/dx, dv/dx, dv/dz, dw/dz");
-        const double gxx = v[5], gyx = v[6], gzx = G[2 * N + gi];
+        const double gxx = v[5], gyx = v[6], gzx = sf[5];
         const double gxy = cr[0], gyy = cr[1], gzy = cr[2];
-        const double gxz = G[6 * N + gi], gyz = v[7], gzz = v[8];
+        const double gxz = sf[6], gyz = v[7], gzz = v[8];
         const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
-        const double tx = mup * G[9 * N + gi], ty = mup * cr[3], tz = mup * G[11 * N + gi];
+        const double tx = mup * sf[7], ty = mup * cr[3], tz = mup * sf[8];
         const double f1 = ph.bulk - 2.0 / 3.0;
         const double dil = gxx + gyy + gzz;
         const double t1 = gyx + gxy, t2 = gzx + gxz, t3 = gzy + gyz;
         const double fX = tx * (2.0 * gxx + f1 * dil) + ty * t1 + tz * t2;
         const double fY = ty * (2.0 * gyy + f1 * dil) + tx * t1 + tz * t3;
         const double fZ = tz * (2.0 * gzz + f1 * dil) + tx * t2 + ty * t3;
-        const double *Pz = a.dQp, *Px = a.dQp + 5 * N;
-        a.dQ[gi] = ((v[0] + Pz[gi]) + Px[gi]) + fX;
-        a.dQ[N + gi] = ((v[1] + Pz[N + gi]) + Px[N + gi]) + fY;
-        a.dQ[2 * N + gi] = ((v[2] + Pz[2 * N + gi]) + Px[2 * N + gi]) + fZ;
-        a.dQ[3 * N + gi] = (v[3] + Pz[3 * N + gi]) + Px[3 * N + gi];
-        a.dQ[4 * N + gi] = ((v[4] + Pz[4 * N + gi]) + Px[4 * N + gi])
+        const double *Pz = sf + 9, *Px = sf + 14;
+        a.dQ[gi] = ((v[0] + Pz[0]) + Px[0]) + fX;
+        a.dQ[N + gi] = ((v[1] + Pz[1]) + Px[1]) + fY;
+        a.dQ[2 * N + gi] = ((v[2] + Pz[2]) + Px[2]) + fZ;
+        a.dQ[3 * N + gi] = (v[3] + Pz[3]) + Px[3];
+        a.dQ[4 * N + gi] = ((v[4] + Pz[4]) + Px[4])
                            + (pr.u * fX + pr.v * fY + pr.w * fZ
                               + mu * (2.0 * (gxx * gxx + gyy * gyy + gzz * gzz) + f1 * dil * dil)
                               + mu * (t1 * t1 + t2 * t2 + t3 * t3));
