@@ -93,11 +93,14 @@ struct Walk {
     long base;            // global index of the thread's first element
     long step;            // global index step per r
     int b, s0;            // x and z: the thread's line and first point
+    int t;                // the thread
 
-    // tile: the tile within its instance; off: the instance's first point
-    DEV Walk(const Geo &g, long tile, long off)
+    // tile: the tile within its instance; off: the instance's first point;
+    // tid: the thread index (threadIdx.x, or a copy the compiler cannot see
+    // through, so that a second walk is computed again instead of kept)
+    DEV Walk(const Geo &g, long tile, long off, int tid)
     {
-        const int t = threadIdx.x;
+        t = tid;
         const long line0 = tile * B;
         if (g.dir == 1) {
             base = off + line0 * n + t;
@@ -123,7 +126,7 @@ struct Walk {
     DEV long at(const Geo &g, int r, int &bb, int &ss) const
     {
         if (g.dir == 1) {
-            const int e = threadIdx.x + NT * r;
+            const int e = t + NT * r;
             bb = e / n;
             ss = e - bb * n;
         } else {

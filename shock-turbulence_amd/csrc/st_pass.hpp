@@ -93,7 +93,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     if (a0.act && !a0.act[slot]) return;          // uniform per workgroup, before any barrier
     PassArgs a;
     slot_params(a0, slot, a);
-    const Walk<M, P, NT> W(g, tile - (long)si * g.tpi, (long)slot * a.ni);
+    const Walk<M, P, NT> W(g, tile - (long)si * g.tpi, (long)slot * a.ni, threadIdx.x);
     constexpr int ss = NT * M;           // slot size: B*M*P with B*P = NT
 
     // 1. load (memory order, coalesced)
@@ -115,13 +115,21 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     const Lane<M, P, NP> L = make_lane<M, P, NP>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
     PASS::template compute<M, P, NP>(g, a, L, out);
+    // The memory-order walk again for the phases after the compute phase,
+    // from a copy of the thread index that the compiler cannot see through:
+    // the load phase's addresses and LDS indices are then computed anew
+    // instead of being kept (in registers, or spilled) through the compute
+    // phase.
+    int tl = threadIdx.x;
+    asm volatile("" : "+v"(tl));
+    const Walk<M, P, NT> Wl(g, tile - (long)si * g.tpi, (long)slot * a.ni, tl);
     if constexpr (has_mid<PASS>::value) {
         static_assert(RE && NO == 0, "a second compute phase works on the slots");
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < M; r++) {
             int b, s;
-            const long gi = W.at(g, r, b, s);
+            const long gi = Wl.at(g, r, b, s);
             const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NS];
 #pragma unroll
@@ -145,13 +153,13 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
 #pragma unroll
             for (int r = 0; r < M; r++) {
                 int b, s;
-                PASS::sfetch_early(g, a, W.at(g, r, b, s), sfe[r]);
+                PASS::sfetch_early(g, a, Wl.at(g, r, b, s), sfe[r]);
             }
         }
 #pragma unroll
         for (int r = 0; r < M; r++) {
             int b, s;
-            const long gi = W.at(g, r, b, s);
+            const long gi = Wl.at(g, r, b, s);
             const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NE + NPF > 0 ? NE + NPF : 1];
 #pragma unroll
@@ -181,7 +189,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             for (int j = 0; j < SFB; j++) {
                 if (r0 + j >= M) break;
                 int b, s;
-                PASS::sfetch(g, a, W.at(g, r0 + j, b, s), sf[j]);
+                PASS::sfetch(g, a, Wl.at(g, r0 + j, b, s), sf[j]);
             }
         }
 #pragma unroll
@@ -189,7 +197,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             if (r0 + j >= M) break;
             const int r = r0 + j;
             int b, s;
-            const long gi = W.at(g, r, b, s);
+            const long gi = Wl.at(g, r, b, s);
             const int k = s % M, c = s / M;
             const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
             constexpr int NV = RE ? NO : NO + NE;
