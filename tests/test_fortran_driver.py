@@ -54,8 +54,9 @@ def test_driver_march_matches_oracle(tmp_path):
     assert relerr(Qg, Qc).max() < 1e-10
     last = [float(x) for x in lines[-1][2:]]
     assert int(lines[-1][1]) == st_c == nsteps
-    assert abs(last[0] - t_c) <= 1e-12 * t_c
-    assert abs(last[1] - dts[-1]) <= 1e-11 * dts[-1]
+    # dt follows the state (1e-10 above); the time is the sum of the dts
+    assert abs(last[0] - t_c) <= 1e-10 * t_c
+    assert abs(last[1] - dts[-1]) <= 1e-10 * dts[-1]
     maxdil, ens = o.compute_grad(Qc)
     d = o.diagnostics(Qc)
     want = [d[3], maxdil, ens, d[0], d[1], d[2], d[4], d[5], d[6]]
