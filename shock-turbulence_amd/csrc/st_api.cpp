@@ -227,6 +227,22 @@ int pass(st_ctx *c, int kind, int dir, st::PassArgs a, double cadd = 1.0)
     while (nt > 256 && ((nt == 1024 && (t.P < 32 || (t.M & (t.M - 1)))) || lot % (nt / t.P))) nt /= 2;
     hipError_t err = st::run_pass(kind, t, nt, dir, c->My, c->Mz, c->Mx, a, c->s);
     if (a.dry) return err == hipSuccess ? 0 : -4;
+    // A shape this grid cannot take (the tile's LDS, say) is refused before
+    // any launch: the next narrower tiles of the same layout, then those of
+    // the other layout.  The defaults are tuned at 256^3 and need not fit
+    // every line length.
+    for (int alt = 0; err == hipErrorInvalidValue && alt < 2; alt++) {
+        const int la = alt ? 3 - lay : lay;
+        const st::DirTab &u = c->th[la + dir];
+        st::PassArgs b = a;
+        b.tab = c->td + la + dir;
+        b.K1 = cadd * c->sx1[dir] * (double)(u.n - 1) * u.op[0].scale;
+        b.K2 = u.s2 * u.op[1].scale;
+        for (int w = alt ? 1024 : nt / 2; w >= 256 && err == hipErrorInvalidValue; w /= 2) {
+            if ((w == 1024 && (u.P < 32 || (u.M & (u.M - 1)))) || lot % (w / u.P)) continue;
+            err = st::run_pass(kind, u, w, dir, c->My, c->Mz, c->Mx, b, c->s);
+        }
+    }
     if (prof) {
         prof_end(c, e);
         c->prof_fields[pk] = kind_fields(kind, dir, a);

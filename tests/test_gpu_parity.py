@@ -4,6 +4,7 @@ Tolerances are norm-wise relative per field, max|gpu - cpu| / max|cpu|:
   compact D1, D2, Gaussian         1e-13
   compact filter (cond ~ 2e3)      1e-12
   one RHS / Hyper_RHS / one stage  1e-12
+  hyper coefficients               1e-11 (x (n/256)^4 for lines n > 256)
   20 steps                         1e-10
   200 steps                        1e-9   (the north-star bar)
 The GPU differs from the oracle only by rounding: fused multiply-adds,
@@ -92,8 +93,14 @@ def test_d1_sinusoid(solvers, shape):
             assert np.abs(s.dfd(d, F) - ex).max() < 1e-13 * max(1, k)
 
 
+# x lines of 512 and 448 points: the default launch shapes of some passes
+# (1024-thread tiles) do not fit them, so the passes fall back to a tile the
+# grid takes
+WIDE_X = [(512, 16, 16), (448, 16, 16)]
+
+
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG + THREE + [(40, 56, 112)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)] + LONG + THREE + [(40, 56, 112)] + WIDE_X)
 def test_rhs(solvers, shape, hyper):
     _log("rhs", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
@@ -110,9 +117,12 @@ def test_rhs(solvers, shape, hyper):
     _log("  full rhs relerr", e)
     assert e.max() < 1e-12
     if hyper:
+        # the coefficients come from fourth derivatives (Lap Lap |S|, Lap Lap
+        # e), whose rounding grows as h^-4: the bar scales with the longest
+        # line (the right-hand side built from them agrees to 1e-12 above)
         eh = relerr(s.hyper(), o.hyper())
         _log("  hyper coeff relerr", eh)
-        assert eh.max() < 1e-11
+        assert eh.max() < 1e-11 * max(1.0, (max(shape) / 256.0) ** 4)
 
 
 def test_rhs_turbulence_ic(solvers):
@@ -145,7 +155,7 @@ def test_reductions_and_fix(solvers):
 
 
 @pytest.mark.parametrize("hyper", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32)] + LONG + THREE[:2])
+@pytest.mark.parametrize("shape", [(32, 32, 32)] + LONG + THREE[:2] + WIDE_X[:1])
 def test_stage(solvers, shape, hyper):
     _log("stage", shape, "hyper", hyper)
     s, o = solvers(shape, Re=80.0, hyper_param=hyper)
