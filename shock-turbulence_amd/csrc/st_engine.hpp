@@ -448,7 +448,8 @@ DEV double line_value(const double (&x)[M], const Lane<M, P, NP> &L, int row)
     }
 }
 
-// Boundary correction x -= Z q, q = G yb (the np_* tables of OpTab).
+// Boundary correction x -= Z q, q = G yb (the np_* tables of OpTab; Z
+// itself is applied through the recurrences, below).
 template <int MB, int M, int P, bool NP>
 DEV void np_fix(double (&x)[M], const OpTab &T, const Lane<M, P, NP> &L)
 {
@@ -467,12 +468,27 @@ DEV void np_fix(double (&x)[M], const OpTab &T, const Lane<M, P, NP> &L)
         for (int t = 1; t < 10; t++) s = fma(T.npG[j][t], yb[t], s);
         q[j] = s;
     }
-    const double *Z = T.npZ + L.c;
+    // Z q = U^-1 L^-1 (scale sum_j q_j e_(row j)), the boundary rows being
+    // the first and last MB/2 of the line: one more solve of a right-hand
+    // side that is zero away from them, instead of the MB columns of Z per
+    // point (which, read once per kernel, would hold MB*M registers per
+    // operator, and read at each use, wait on memory in every solve)
+    constexpr int K = MB == 6 ? 2 : 1, H = MB / 2;
+    double z[M];
 #pragma unroll
     for (int k = 0; k < M; k++) {
+        const int i = L.c * M + k;
+        double v = 0.0;
 #pragma unroll
-        for (int j = 0; j < MB; j++) x[k] = fma(-q[j], Z[(j * M + k) * P], x[k]);
+        for (int j = 0; j < H; j++) {
+            v = i == j ? T.scale * q[j] : v;
+            v = i == n - H + j ? T.scale * q[H + j] : v;
+        }
+        z[k] = v;
     }
+    solve<K, M>(z, T, L);
+#pragma unroll
+    for (int k = 0; k < M; k++) x[k] -= z[k];
 }
 
 // Closure rows of the right-hand side.  OP 0: first derivative
