@@ -162,9 +162,11 @@ int gj_solve(int m, LD a[6][6], LD g[6][10])
     return 0;
 }
 
-// Boundary-correction tables of one operator (the np_* fields of OpTab),
-// built from the rounded recurrence coefficients the kernels use.
-int fill_np(OpTab *t, int op, double alpha, double beta, int n, int M, int P)
+// Boundary-correction table G of one operator (the np_* fields of OpTab),
+// built from the rounded recurrence coefficients the kernels use (the
+// columns of Z only on the host: the kernels apply Z through the same
+// recurrences).
+int fill_np(OpTab *t, int op, double alpha, double beta, int n)
 {
     if (n < 16 || n > 512) return -1;
     const int m = op == 2 ? 6 : 4;
@@ -224,11 +226,8 @@ int fill_np(OpTab *t, int op, double alpha, double beta, int n, int M, int P)
     }
     if (gj_solve(m, IVZ, G)) return -2;
     t->np_m = m;
-    for (int j = 0; j < m; j++) {
+    for (int j = 0; j < m; j++)
         for (int tt = 0; tt < 10; tt++) t->npG[j][tt] = (double)G[j][tt];
-        for (int c = 0; c < P; c++)
-            for (int k = 0; k < M; k++) t->npZ[(j * M + k) * P + c] = (double)Z[(size_t)j * n + c * M + k];
-    }
     return 0;
 }
 
@@ -284,9 +283,9 @@ int build_dirtab(DirTab *t, int n, int M, int P, int B, double sx1, double sx2,
     if (fill_op(&t->op[2], f_alpha, f_beta, n, M, P)) return -3;
     t->nonper = nonper ? 1 : 0;
     if (nonper) {
-        if (fill_np(&t->op[0], 0, d1_alpha, 0.0, n, M, P)) return -4;
-        if (fill_np(&t->op[1], 1, d2_alpha, 0.0, n, M, P)) return -5;
-        if (fill_np(&t->op[2], 2, f_alpha, f_beta, n, M, P)) return -6;
+        if (fill_np(&t->op[0], 0, d1_alpha, 0.0, n)) return -4;
+        if (fill_np(&t->op[1], 1, d2_alpha, 0.0, n)) return -5;
+        if (fill_np(&t->op[2], 2, f_alpha, f_beta, n)) return -6;
     }
     t->s1 = sx1 * (double)(n - 1);
     t->s2 = (double)((n - 1) * (n - 1)) * sx2;

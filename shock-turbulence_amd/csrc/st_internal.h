@@ -31,11 +31,11 @@ struct OpTab {
     // A = Ah + sum_j e_(i_j) v_j^T over the np_m boundary rows i_j, and
     //   x = y - Z q,  q = G yb,  y = Ah^-1 b,
     // with Z = Ah^-1 [e_(i_j)] and G = (I + V^T Z)^-1 V^T restricted to the
-    // ten boundary values yb = (y_0..y_4, y_(n-5)..y_(n-1)) (Woodbury).
+    // ten boundary values yb = (y_0..y_4, y_(n-5)..y_(n-1)) (Woodbury); Z q
+    // is one more solve of the boundary rows, so only G is tabled.
     int np_m;             // boundary rows (4: first/second derivative, 6: filter)
     int pad2_;
     double npG[6][10];
-    double npZ[6 * 512];  // column j at row c*M + k: npZ[(j*M + k)*P + c]
 };
 
 // Per-direction operator tables and stencil factors.
