@@ -17,17 +17,31 @@
  *                                               sinusoids (to roundoff),
  *                                               6th-order refinement,
  *                                               D2(const)=0, T(pi)=0, G(0)=1
- *   RHS, Hyper_RHS                              uniform state -> 0; mass
- *                                               conservation; axis-swap
- *                                               symmetry; manufactured
- *                                               Euler solution at 6th order;
- *                                               momentum/energy conservation
- *                                               of the hyper terms
- *   Compute_hyper (coefficients)                composite: parity unpinned
- *                                               beyond symmetry/uniform-state
- *                                               checks (no reference vectors
- *                                               exist and the Fortran cannot
- *                                               be built in this image)
+ *   RHS, /GRAD/, Compute_coeff1,                a second restatement from
+ *   Compute_hyper, Hyper_RHS, one stage          the reference text with every
+ *                                               periodic compact operator
+ *                                               applied through its exact
+ *                                               discrete symbol in Fourier
+ *                                               space (tests/spectral_ref.py):
+ *                                               agreement to 1e-15..8e-15
+ *                                               (3e-13 for a stage, through
+ *                                               the filter's band solve) on
+ *                                               multi-mode states with the
+ *                                               artificial transport on; plus
+ *                                               closed forms of the sensor,
+ *                                               uniform state -> 0, mass,
+ *                                               momentum and energy
+ *                                               conservation, axis-swap
+ *                                               symmetry, a manufactured
+ *                                               Euler solution at 6th order,
+ *                                               and a planted-mistake test
+ *   non-periodic closure rows, serial solve,    polynomial exactness of each
+ *   mirrored Gaussian                           closure row (and failure one
+ *                                               degree above), dense LU
+ *                                               solves, numpy's reflect-padded
+ *                                               convolution
+ *   libm at the boundary (tanh, sqrt, pow)      parity unpinned (no pinned
+ *                                               runtime in the reference)
  *   RK stage / step                             4th-order convergence on a
  *                                               linear ODE through the same
  *                                               coefficient tables
