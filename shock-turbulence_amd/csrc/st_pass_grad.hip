@@ -92,13 +92,15 @@ struct PassGrad {
         }
     }
     // Accumulate into row i of out (one of the rows in MASK) with static
-    // register indices.
+    // register indices: every row in MASK takes w_r v, w_r = 1 for row i and
+    // 0 otherwise (i is the same on every lane, so w_r is a scalar and each
+    // row costs one fused multiply-add, where a select costs three).
     template <int MASK, int M>
     DEV static void acc(double (&out)[5][M], int i, int k, double v)
     {
 #pragma unroll
         for (int r = 0; r < 5; r++)
-            if ((MASK >> r) & 1) out[r][k] += (i == r) ? v : 0.0;
+            if ((MASK >> r) & 1) out[r][k] = fma(i == r ? 1.0 : 0.0, v, out[r][k]);
     }
     // Operator order: fluxes and the mass flux first, while u, v, w, T and
     // rho are all needed; then T, whose slot takes dT/dx_d and rho's slot
