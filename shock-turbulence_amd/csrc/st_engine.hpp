@@ -58,6 +58,15 @@ DEV long tile_of(const Geo &g, long wg)
 // f = k (P/M) + b (4b for M = 4).  (M = 3 or 6: rows rotate by b mod M; any
 // rotation and swizzle is a bijection of the block, only the bank spread
 // depends on them.)
+// The additive layout (AL, M a power of two) drops the rotation and the
+// row-dependent part of the key: point s = c*M + k of line b sits at
+// b*n + k*P + (c ^ g(b)).  A lane's points then lie at one base plus
+// constant offsets, so one address register serves all of its rows (the
+// swizzled layout needs one per row, which the passes with the most live
+// values spill).  The compute phase and the staging of x and z tiles of 16
+// or more lines stay free of bank conflicts; the staging of y tiles and of
+// narrower x and z tiles takes up to M-way conflicts.  Each pass names the
+// layout that measured faster for it (LDS_ADD).
 template <int M>
 HD int lds_g(int b, int B)
 {
@@ -69,10 +78,11 @@ HD int lds_g(int b, int B)
     return (b & ((1 << lo) - 1)) | ((b >> lo) << (lo + kb));
 }
 
-template <int M>
+template <int M, bool AL = false>
 HD int lds_idx(int b, int k, int c, int P, int g)
 {
     if constexpr ((M & (M - 1)) == 0) {
+        if constexpr (AL) return b * (M * P) + k * P + (c ^ (g & (P - 1)));
         const int f = P >= 16 ? (16 / M) * k + g : k * (P / M) + (M == 4 ? 4 * b : b);
         return b * (M * P) + (((k + b) * P + (c ^ (f & (P - 1)))) & (M * P - 1));
     } else {
@@ -139,26 +149,28 @@ struct Walk {
 
 // Per-lane view during the compute phase; P lanes per line.  NP: the line
 // is non-periodic (closure rows at both ends, no wrap).
-template <int M, int P_, bool NP_ = false>
+template <int M, int P_, bool NP_ = false, bool AL_ = false>
 struct Lane {
     static constexpr int P = P_;
     static constexpr bool NP = NP_;
+    static constexpr bool AL = AL_;   // additive LDS layout
     int c, seg, b;         // chunk index, first lane of the line, line in tile
     int g;                 // the line's column key (lds_g)
     double *lds;
     int ss;                // slot size (doubles)
 
-    DEV double ld(int slot, int k) const { return lds[slot * ss + lds_idx<M>(b, k, c, P, g)]; }
-    DEV void st(int slot, int k, double v) const { lds[slot * ss + lds_idx<M>(b, k, c, P, g)] = v; }
+    DEV double ld(int slot, int k) const { return lds[slot * ss + lds_idx<M, AL>(b, k, c, P, g)]; }
+    DEV void st(int slot, int k, double v) const { lds[slot * ss + lds_idx<M, AL>(b, k, c, P, g)] = v; }
     DEV int src(int dc) const { return seg + ((c + dc) & (P - 1)); }
 };
 
 // ss = B*M*P = NT*M doubles: a compile-time constant for the kernel, so the
-// slot offsets fold into the LDS instructions' immediate offsets.
-template <int M, int P, bool NP = false>
-DEV Lane<M, P, NP> make_lane(double *lds, int ss)
+// slot offsets fold into the LDS instructions' immediate offsets (and, with
+// the additive layout, the row offsets too).
+template <int M, int P, bool NP = false, bool AL = false>
+DEV Lane<M, P, NP, AL> make_lane(double *lds, int ss)
 {
-    Lane<M, P, NP> L;
+    Lane<M, P, NP, AL> L;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     L.c = lane & (P - 1);
@@ -175,8 +187,8 @@ DEV Lane<M, P, NP> make_lane(double *lds, int ss)
 // images f(-m) = f(m), f(n-1+m) = f(n-1-m) (the symmetry of
 // src/filters_nonperiodic1d.f:830-943); the compact operators overwrite
 // their closure rows and never read them.
-template <int M, int P, bool NP>
-DEV void window(const double (&v)[M], const Lane<M, P, NP> &L, double (&w)[M + 2 * HALO])
+template <int M, int P, bool NP, bool AL>
+DEV void window(const double (&v)[M], const Lane<M, P, NP, AL> &L, double (&w)[M + 2 * HALO])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) w[HALO + k] = v[k];
@@ -375,8 +387,8 @@ DEV void wave_scan(double (&e)[K], double (&tot)[K], const OpTab &T, int c)
 // sides (the scale 1+rho^2 or 1/kappa already folded in) and returns x.
 // Order 1: x = (1 - rho S^-1)^-1 (1 - rho S)^-1 r,
 // order 2: x = (1 - a1 S^-1 - a2 S^-2)^-1 (1 - a1 S - a2 S^2)^-1 r.
-template <int K, int M, int P, bool NP>
-DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P, NP> &L)
+template <int K, int M, int P, bool NP, bool AL>
+DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P, NP, AL> &L)
 {
     const double a1 = T.a1, a2 = T.a2;
     const int c = L.c;
@@ -436,8 +448,8 @@ DEV void solve(double (&r)[M], const OpTab &T, const Lane<M, P, NP> &L)
 // ---- non-periodic lines ---------------------------------------------------
 
 // Value at row ROW of this lane's line after a solve (every lane gets it).
-template <int M, int P, bool NP>
-DEV double line_value(const double (&x)[M], const Lane<M, P, NP> &L, int row)
+template <int M, int P, bool NP, bool AL>
+DEV double line_value(const double (&x)[M], const Lane<M, P, NP, AL> &L, int row)
 {
     const int src = row / M;
     if constexpr (P == 64) {
@@ -450,8 +462,8 @@ DEV double line_value(const double (&x)[M], const Lane<M, P, NP> &L, int row)
 
 // Boundary correction x -= Z q, q = G yb (the np_* tables of OpTab; Z
 // itself is applied through the recurrences, below).
-template <int MB, int M, int P, bool NP>
-DEV void np_fix(double (&x)[M], const OpTab &T, const Lane<M, P, NP> &L)
+template <int MB, int M, int P, bool NP, bool AL>
+DEV void np_fix(double (&x)[M], const OpTab &T, const Lane<M, P, NP, AL> &L)
 {
     constexpr int n = M * P;
     double yb[10];
@@ -540,8 +552,8 @@ DEV double closure_row(int r, const F &f)
 // Overwrite the closure rows (the first NB and last NB of the line) of the
 // right-hand side out = K * stencil.  The chunks that hold them are
 // enumerated at compile time so every window index is a constant.
-template <int OP, int M, int P, bool NP>
-DEV void closure_rows(const double (&w)[M + 2 * HALO], double K, const Lane<M, P, NP> &L, double (&out)[M])
+template <int OP, int M, int P, bool NP, bool AL>
+DEV void closure_rows(const double (&w)[M + 2 * HALO], double K, const Lane<M, P, NP, AL> &L, double (&out)[M])
 {
     constexpr int n = M * P, NB = OP == 2 ? 4 : 3;
     // (selects, not branches: every index stays a constant)
@@ -576,9 +588,9 @@ DEV void closure_rows(const double (&w)[M + 2 * HALO], double K, const Lane<M, P
 // ---- operators on a window -----------------------------------------------
 
 // First derivative: K = cadd*sx(1)*(N-1)*(1+rho^2).
-template <int M, int P, bool NP>
+template <int M, int P, bool NP, bool AL>
 DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M, P, NP> &L, double (&out)[M])
+               const Lane<M, P, NP, AL> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -591,9 +603,9 @@ DEV void op_d1(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
 }
 
 // Second derivative: K = (N-1)^2*sx(2)*(1+rho^2).
-template <int M, int P, bool NP>
+template <int M, int P, bool NP, bool AL>
 DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-               const Lane<M, P, NP> &L, double (&out)[M])
+               const Lane<M, P, NP, AL> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -606,9 +618,9 @@ DEV void op_d2(const double (&w)[M + 2 * HALO], const Coef &C, double K, const O
 }
 
 // Eighth-order compact filter: K = 1/kappa.
-template <int M, int P, bool NP>
+template <int M, int P, bool NP, bool AL>
 DEV void op_filt(const double (&w)[M + 2 * HALO], const Coef &C, double K, const OpTab &T,
-                 const Lane<M, P, NP> &L, double (&out)[M])
+                 const Lane<M, P, NP, AL> &L, double (&out)[M])
 {
 #pragma unroll
     for (int k = 0; k < M; k++) {

@@ -53,6 +53,13 @@ struct sfe_n { static constexpr int value = 0; };
 template <class T>
 struct sfe_n<T, std::void_t<decltype(T::NSFE)>> { static constexpr int value = T::NSFE; };
 
+// LDS_ADD of a pass (false when it names none): the additive LDS layout
+// (lds_idx) instead of the swizzled one.
+template <class T, class = void>
+struct lds_add { static constexpr bool value = false; };
+template <class T>
+struct lds_add<T, std::void_t<decltype(T::LDS_ADD)>> { static constexpr bool value = T::LDS_ADD; };
+
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
 template <class T>
@@ -83,6 +90,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     static_assert(NO > 0 || RE, "a pass without late outputs stores through store_early");
     static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
     constexpr int NPF = PASS::NPF;
+    constexpr bool AL = lds_add<PASS>::value;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
     constexpr int NCR = carry_n<PASS>::value;
@@ -105,14 +113,14 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
         PASS::load(g, a, gi, v);
         if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
-        const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
+        const int li = lds_idx<M, AL>(b, k, c, P, lds_g<M>(b, NT / P));
 #pragma unroll
         for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
     }
     __syncthreads();
 
     // 2. compute (line mapping)
-    const Lane<M, P, NP> L = make_lane<M, P, NP>(lds, ss);
+    const Lane<M, P, NP, AL> L = make_lane<M, P, NP, AL>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
     PASS::template compute<M, P, NP>(g, a, L, out);
     // The memory-order walk again for the phases after the compute phase,
@@ -130,7 +138,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
         for (int r = 0; r < M; r++) {
             int b, s;
             const long gi = Wl.at(g, r, b, s);
-            const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
+            const int li = lds_idx<M, AL>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NS];
 #pragma unroll
             for (int q = 0; q < NS; q++) v[q] = lds[q * ss + li];
@@ -160,7 +168,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
         for (int r = 0; r < M; r++) {
             int b, s;
             const long gi = Wl.at(g, r, b, s);
-            const int li = lds_idx<M>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
+            const int li = lds_idx<M, AL>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
             double v[NE + NPF > 0 ? NE + NPF : 1];
 #pragma unroll
             for (int j = 0; j < NE; j++) v[j] = lds[j * ss + li];
@@ -199,7 +207,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             int b, s;
             const long gi = Wl.at(g, r, b, s);
             const int k = s % M, c = s / M;
-            const int li = lds_idx<M>(b, k, c, P, lds_g<M>(b, NT / P));
+            const int li = lds_idx<M, AL>(b, k, c, P, lds_g<M>(b, NT / P));
             constexpr int NV = RE ? NO : NO + NE;
             constexpr int NK = keep_n<PASS>::value;
             static_assert(NK == 0 || (RE && NS - NK >= NO && NS - NK >= NE), "kept slots hold no results");
@@ -243,8 +251,8 @@ struct NoReuse {
 // ---------------------------------------------------------------------
 DEV double qf(const PassArgs &a, int comp, long gi) { return a.Q[comp * a.N + gi]; }
 
-template <int M, int P, bool NP>
-DEV void qown(const Lane<M, P, NP> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
+template <int M, int P, bool NP, bool AL>
+DEV void qown(const Lane<M, P, NP, AL> &L, int k, double &q1, double &q2, double &q3, double &q4, double &q5)
 {
     q1 = L.ld(0, k);
     q2 = L.ld(1, k);

@@ -35,8 +35,8 @@ struct PassFilterQ {
             }
         }
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
         const OpTab &T = a.tab->op[2];
 #pragma unroll 1

@@ -49,6 +49,11 @@ struct PassGrad {
     static constexpr int MIN_WAVES = MW;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
+    // the additive LDS layout for the x and y sweeps (fewer address
+    // registers: the x pass at 8 points per lane no longer spills, 0.73 ->
+    // 0.66 ms at 256^3, and the y pass runs 2% faster); the swizzled one for
+    // z, whose 4-line tiles would take 4-way bank conflicts in the staging
+    static constexpr bool LDS_ADD = NMX != 2;
     // CROSS: the store phases' global reads (sfetch_early: the z and x
     // partials of the Laplacian of e; sfetch: the state, four x and z
     // derivatives, and the z and x partial sums) go out for all the
@@ -71,8 +76,8 @@ struct PassGrad {
         for (int m = 0; m < NMX; m++) v[5 + m] = a.G[msrc(m) * a.N + gi];
     }
     // The mixed-derivative terms, first (the velocities are still staged).
-    template <int M, int P, bool NP>
-    DEV static void compute_mix(const PassArgs &a, const Lane<M, P, NP> &L, const double (&mu)[M], double (&out)[5][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute_mix(const PassArgs &a, const Lane<M, P, NP, AL> &L, const double (&mu)[M], double (&out)[5][M])
     {
         const double cb = a.ph.bulk + 1.0 / 3.0;
 #pragma unroll 1
@@ -107,8 +112,8 @@ struct PassGrad {
     // the Laplacian of e; then each velocity, whose slot takes its
     // derivative.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations.
-    template <int M, int P, bool NP>
-    DEV static void compute_wide(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute_wide(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -201,8 +206,8 @@ struct PassGrad {
     // the Laplacian of e.  The loop bodies are free of branches so that the
     // accumulators stay in place across iterations, and each loop iteration
     // keeps one window and one solve live at a time.
-    template <int M, int P, bool NP>
-    DEV static void compute_lean(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute_lean(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[5][M])
     {
         const Phys &ph = a.ph;
         const int d = g.dir;
@@ -210,11 +215,14 @@ struct PassGrad {
         double mu[M];
 #pragma unroll
         for (int k = 0; k < M; k++) {
-            mu[k] = visc(ph, L.ld(3, k));
 #pragma unroll
             for (int i = 0; i < 5; i++) out[i][k] = 0.0;
         }
-        if constexpr (NMX > 0) compute_mix<M, P, NP>(a, L, mu, out);
+        if constexpr (NMX > 0) {
+#pragma unroll
+            for (int k = 0; k < M; k++) mu[k] = visc(ph, L.ld(3, k));
+            compute_mix<M, P, NP>(a, L, mu, out);
+        }
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
         for (int it = 0; it < 4; it++) {
@@ -250,6 +258,12 @@ struct PassGrad {
                 out[2][k] += t * L.ld(2, k);
                 out[3][k] = -sk[k];
             }
+        }
+        // the viscosity, first needed here (without mixed terms) so that it
+        // is not held through the flux solves
+        if constexpr (NMX == 0) {
+#pragma unroll
+            for (int k = 0; k < M; k++) mu[k] = visc(ph, L.ld(3, k));
         }
         // u, v, w (u_d last): second, then first derivatives
 #pragma unroll 1
@@ -298,8 +312,8 @@ struct PassGrad {
         }
     }
 
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[5][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[5][M])
     {
         if constexpr (LEAN) compute_lean<M, P, NP>(g, a, L, out);
         else compute_wide<M, P, NP>(g, a, L, out);
@@ -404,8 +418,7 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
     // x first (no mixed terms), z with two, y last with four and the
     // pointwise assembly
     if (kind == K_GRAD && dir == 0) {
-        if (nt == 1024) return launch<PassGrad<1, true, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
-        return launch<PassGrad<1, false, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
+        return launch<PassGrad<1, true, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
     }
     if (kind == K_GRAD && dir == 2) {
         if (nt == 1024) return launch<PassGrad<1, true, false, 2>, false>(t, nt, dir, My, Mz, Mx, a, s);
@@ -421,7 +434,7 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
 // The 256-thread variants (those of the plain path below 1024 threads).
 hipError_t launch_grad_ragged(int kind, const RaggedSpec &sp, hipStream_t s)
 {
-    if (kind == K_GRAD && sp.dir == 0) return launch_ragged<PassGrad<1, false, false, 0>>(sp, s);
+    if (kind == K_GRAD && sp.dir == 0) return launch_ragged<PassGrad<1, true, false, 0>>(sp, s);
     if (kind == K_GRAD && sp.dir == 2) return launch_ragged<PassGrad<1, false, false, 2>>(sp, s);
     if (kind == K_GRAD_CROSS && sp.dir == 1) return launch_ragged<PassGrad<2, false, true, 4>>(sp, s);
     return hipErrorInvalidValue;

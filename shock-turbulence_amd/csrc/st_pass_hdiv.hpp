@@ -63,8 +63,8 @@ struct PassHyperDiv {
         v[2] = tz;
         v[3] = uh * tx + vh * ty + wh * tz + kap * G[(9 + g.dir) * N + gi];
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
 #pragma unroll 1
         for (int m = 0; m < 4; m++) {

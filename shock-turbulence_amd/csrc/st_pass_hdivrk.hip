@@ -36,8 +36,8 @@ struct PassHdivRkf {
         for (int c = 0; c < 4; c++) v[c] = t[c];
         v[4] = 0.0;
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L, double (&o)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&o)[1][M])
     {
         PassHyperDiv::compute<M, P, NP>(g, a, L, o);
     }
@@ -65,8 +65,8 @@ struct PassHdivRkf {
             v[c] = fma(a.rk_b, qw, q);
         }
     }
-    template <int M, int P, bool NP>
-    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L)
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L)
     {
         const OpTab &T = a.tab->op[2];
 #pragma unroll 1

@@ -10,8 +10,8 @@ namespace st {
 struct PassLap2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.L2S[gi]; }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -23,8 +23,8 @@ struct PassLap2 : NoReuse {
 };
 
 // D2 along the sweep of the first staged slot, in place.
-template <int M, int P, bool NP>
-DEV void d2_slot(const PassArgs &a, const Lane<M, P, NP> &L, int m)
+template <int M, int P, bool NP, bool AL>
+DEV void d2_slot(const PassArgs &a, const Lane<M, P, NP, AL> &L, int m)
 {
     double f[M], w[M + 2 * HALO], r[M];
 #pragma unroll
@@ -57,8 +57,8 @@ struct PassLapSELap2 {
         v[0] = a.S[gi];
         v[1] = a.L2e[gi];
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
 #pragma unroll 1
         for (int m = 0; m < 2; m++) d2_slot<M, P, NP>(a, L, m);
@@ -71,8 +71,8 @@ struct PassLapSELap2 {
         v[0] = l2s;
         v[1] = 0.0;
     }
-    template <int M, int P, bool NP>
-    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L)
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L)
     {
         d2_slot<M, P, NP>(a, L, 0);
     }
@@ -102,8 +102,8 @@ struct PassLap2Gauss {
         v[0] = a.L2S[gi];
         v[1] = 0.0;
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
         d2_slot<M, P, NP>(a, L, 0);
     }
@@ -118,8 +118,8 @@ struct PassLap2Gauss {
         v[0] = q4 * d4S;
         v[1] = fac3 * d4e;
     }
-    template <int M, int P, bool NP>
-    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L)
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute2(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L)
     {
 #pragma unroll 1
         for (int m = 0; m < 2; m++) {
@@ -151,8 +151,8 @@ struct PassGaussEta : NoReuse {
         v[0] = a.eta[gi];
         v[1] = a.eta[a.N + gi];
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[2][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[2][M])
     {
 #pragma unroll
         for (int m = 0; m < 2; m++) {
@@ -195,8 +195,8 @@ struct PassGaussHdiv {
         v[2] = 0.0;
         v[3] = 0.0;
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
 #pragma unroll
         for (int m = 0; m < 2; m++) {
@@ -222,8 +222,8 @@ struct PassGaussHdiv {
         a.Hc[2 * N + gi] = kap;
         PassHyperDiv::fluxes(g, a, gi, beta, mu, kap, v);
     }
-    template <int M, int P, bool NP>
-    DEV static void compute2(const Geo &g, const PassArgs &a, const Lane<M, P, NP> &L)
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute2(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L)
     {
         double o[1][M];
         PassHyperDiv::compute<M, P, NP>(g, a, L, o);

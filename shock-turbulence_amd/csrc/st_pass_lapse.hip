@@ -30,8 +30,8 @@ struct PassLapSE {
         v[0] = a.S[gi];
         v[1] = a.L2e[gi];
     }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&)[1][M])
     {
 #pragma unroll 1
         for (int m = 0; m < 2; m++) {

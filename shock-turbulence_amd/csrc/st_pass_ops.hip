@@ -10,8 +10,8 @@ namespace st {
 struct PassD1 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -28,8 +28,8 @@ struct PassD1 : NoReuse {
 struct PassD2 : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 1;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO], d2[M];
 #pragma unroll
@@ -54,8 +54,8 @@ struct PassD2 : NoReuse {
 struct PassFilt : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
@@ -69,8 +69,8 @@ struct PassFilt : NoReuse {
 struct PassGauss : NoReuse {
     static constexpr int NSLOT = 1, NOUT = 1, NEARLY = 0;
     DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[1]) { v[0] = a.in0[gi]; }
-    template <int M, int P, bool NP>
-    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP> &L, double (&out)[1][M])
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[1][M])
     {
         double f[M], w[M + 2 * HALO];
 #pragma unroll
