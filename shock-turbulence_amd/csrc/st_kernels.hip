@@ -23,7 +23,7 @@ void pass_default_cfg(int kind, int dir, int *narrow, int *nt)
     struct C { int n, t; };
     C c = {0, 256};
     switch (kind) {
-    case K_GRAD: c = dir == 0 ? C{0, 512} : C{1, 256}; break;
+    case K_GRAD: c = {0, 256}; break;
     case K_GRAD_CROSS: c = {1, 256}; break;
     case K_LAPSE: c = dir == 1 ? C{1, 512} : C{0, 512}; break;
     case K_LAP2: c = {0, 512}; break;

@@ -53,12 +53,22 @@ struct sfe_n { static constexpr int value = 0; };
 template <class T>
 struct sfe_n<T, std::void_t<decltype(T::NSFE)>> { static constexpr int value = T::NSFE; };
 
-// LDS_ADD of a pass (false when it names none): the additive LDS layout
-// (lds_idx) instead of the swizzled one.
-template <class T, class = void>
+// lds_add(M) of a pass (false when it names none): the additive LDS layout
+// (lds_idx) instead of the swizzled one, for the pass at M points per lane.
+template <class T, int M, class = void>
 struct lds_add { static constexpr bool value = false; };
+template <class T, int M>
+struct lds_add<T, M, std::void_t<decltype(T::lds_add(M))>> { static constexpr bool value = T::lds_add(M); };
+
+// NHOLD of a pass (0 when it names none): values per point that load()
+// returns after the NSLOT staged ones and that wait in registers while
+// compute_a() runs on the slots; they are then written into the slots
+// hold_slot(j) (which compute_a() has finished reading) and compute() runs.
+// A pass can so stage more fields over its compute than it has slots.
+template <class T, class = void>
+struct hold_n { static constexpr int value = 0; };
 template <class T>
-struct lds_add<T, std::void_t<decltype(T::LDS_ADD)>> { static constexpr bool value = T::LDS_ADD; };
+struct hold_n<T, std::void_t<decltype(T::NHOLD)>> { static constexpr int value = T::NHOLD; };
 
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
@@ -90,9 +100,12 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     static_assert(NO > 0 || RE, "a pass without late outputs stores through store_early");
     static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
     constexpr int NPF = PASS::NPF;
-    constexpr bool AL = lds_add<PASS>::value;
+    constexpr bool AL = lds_add<PASS, M>::value;
+    constexpr int NH = hold_n<PASS>::value;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
+    static_assert(NH == 0 || PASS::LOAD_UNROLL >= M, "holding needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
+    double hold[M][NH > 0 ? NH : 1];
     constexpr int NCR = carry_n<PASS>::value;
     static_assert(NCR == 0 || (RE && NO > 0), "a carry goes from store_early to store");
     double cr[M][NCR > 0 ? NCR : 1];
@@ -109,19 +122,34 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     for (int r = 0; r < M; r++) {
         int b, s;
         const long gi = W.at(g, r, b, s);
-        double v[NS > 0 ? NS : 1];
+        double v[NS + NH > 0 ? NS + NH : 1];
         PASS::load(g, a, gi, v);
         if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
         const int li = lds_idx<M, AL>(b, k, c, P, lds_g<M>(b, NT / P));
 #pragma unroll
         for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
+#pragma unroll
+        for (int j = 0; j < NH; j++) hold[r][j] = v[NS + j];
     }
     __syncthreads();
 
     // 2. compute (line mapping)
     const Lane<M, P, NP, AL> L = make_lane<M, P, NP, AL>(lds, ss);
     double out[NO > 0 ? NO : 1][M];
+    if constexpr (NH > 0) {
+        PASS::template compute_a<M, P, NP>(g, a, L, out);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            int b, s;
+            (void)W.at(g, r, b, s);
+            const int li = lds_idx<M, AL>(b, s % M, s / M, P, lds_g<M>(b, NT / P));
+#pragma unroll
+            for (int j = 0; j < NH; j++) lds[PASS::hold_slot(j) * ss + li] = hold[r][j];
+        }
+        __syncthreads();
+    }
     PASS::template compute<M, P, NP>(g, a, L, out);
     // The memory-order walk again for the phases after the compute phase,
     // from a copy of the thread index that the compiler cannot see through:
@@ -341,10 +369,9 @@ static hipError_t launch_m(int M, int P, const Geo &g, const PassArgs &a, hipStr
 // t holds the tables of the chosen layout (M points per lane, P lanes per
 // line); nt threads per workgroup give B = nt/P lines per tile.
 // XL = false for passes that never sweep x (no non-periodic instances).
-template <class PASS, bool XL = true>
-static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+// The tile geometry of a launch; false when nt threads cannot tile the lines.
+static inline bool pass_geo(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, Geo &g)
 {
-    Geo g;
     g.dir = dir;
     g.n = t.n;
     g.My = My;
@@ -352,11 +379,28 @@ static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int M
     g.Mx = Mx;
     g.P = t.P;
     g.B = nt / t.P;
-    if (g.B < 1 || g.B * t.P != nt) return hipErrorInvalidValue;
+    if (g.B < 1 || g.B * t.P != nt) return false;
     const long lot = (long)My * Mz * Mx / g.n;          // lines per instance
-    if (lot % g.B || a.ni != (long)My * Mz * Mx || a.nslot < 1) return hipErrorInvalidValue;
+    if (lot % g.B || a.ni != (long)My * Mz * Mx || a.nslot < 1) return false;
     g.tpi = lot / g.B;
     g.ntiles = g.tpi * a.nslot;
+    return true;
+}
+
+// One periodic layout and tile width of a pass (M, P, NT fixed).
+template <class PASS, int M, int P, int NT>
+static hipError_t launch_at(const DirTab &t, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+{
+    Geo g;
+    if (t.M != M || t.P != P || t.nonper || !pass_geo(t, NT, dir, My, Mz, Mx, a, g)) return hipErrorInvalidValue;
+    return launch_t<M, P, NT, false, PASS>(g, a, s);
+}
+
+template <class PASS, bool XL = true>
+static hipError_t launch(const DirTab &t, int nt, int dir, int My, int Mz, int Mx, const PassArgs &a, hipStream_t s)
+{
+    Geo g;
+    if (!pass_geo(t, nt, dir, My, Mz, Mx, a, g)) return hipErrorInvalidValue;
     // non-periodic lines are the x lines of the shock-tube link set
     if (t.nonper && dir == 0) {
         if constexpr (XL) {

@@ -38,9 +38,18 @@ namespace st {
 // each times (1/3 + bulk) mu into the momentum component it belongs to and,
 // times that velocity, into the energy.  The x sweep runs first, so the z
 // and y sweeps find the derivatives they need.
-template <int MW, bool LEAN, bool CROSS, int NMX>
+// TWO (the z sweep at 8 points per lane): the two mixed inputs are staged in
+// the slots of v and rho, whose values wait in registers (NHOLD) until the
+// mixed terms are done (compute_a); five slots instead of seven, so that
+// 8-line tiles fit two workgroups per CU (0.70 -> 0.645 ms at 256^3; at 4
+// points per lane the extra phase costs more than it gives, 0.70 -> 0.79).
+template <int MW, bool LEAN, bool CROSS, int NMX, bool TWO_ = false>
 struct PassGrad {
-    static constexpr int NSLOT = 5 + NMX, NOUT = 5, NEARLY = 5;
+    static constexpr bool TWO = TWO_;
+    static_assert(!TWO || NMX == 2, "two-round staging is the z sweep's");
+    static constexpr int NSLOT = TWO ? 5 : 5 + NMX, NOUT = 5, NEARLY = 5;
+    static constexpr int NHOLD = TWO ? 2 : 0;
+    HD static constexpr int hold_slot(int j) { return j == 0 ? 1 : 4; }
     static constexpr bool REUSE = true;
     static constexpr int NCARRY = CROSS ? 4 : 0;
     // CROSS: the staged inputs of the mixed terms (du/dx, dv/dx, dv/dz,
@@ -50,10 +59,11 @@ struct PassGrad {
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
     // the additive LDS layout for the x and y sweeps (fewer address
-    // registers: the x pass at 8 points per lane no longer spills, 0.73 ->
-    // 0.66 ms at 256^3, and the y pass runs 2% faster); the swizzled one for
-    // z, whose 4-line tiles would take 4-way bank conflicts in the staging
-    static constexpr bool LDS_ADD = NMX != 2;
+    // registers: the x pass at 8 points per lane no longer spills, and the y
+    // pass runs 2% faster) and for z at 8 points per lane; the swizzled one
+    // for z at 4 points per lane, whose 4-line tiles would take 4-way bank
+    // conflicts in the staging
+    HD static constexpr bool lds_add(int m) { return NMX != 2 || m >= 8; }
     // CROSS: the store phases' global reads (sfetch_early: the z and x
     // partials of the Laplacian of e; sfetch: the state, four x and z
     // derivatives, and the z and x partial sums) go out for all the
@@ -64,16 +74,23 @@ struct PassGrad {
     HD static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
     // momentum component that D1 of mixed input m feeds
     HD static constexpr int mdst(int m) { return NMX == 4 ? (m == 1 ? 0 : m == 2 ? 2 : 1) : (m == 0 ? 2 : 0); }
-    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT])
+    // slot of mixed input m
+    HD static constexpr int mslot(int m) { return TWO ? (m == 0 ? 1 : 4) : 5 + m; }
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT + NHOLD])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
         v[0] = pr.u;
-        v[1] = pr.v;
         v[2] = pr.w;
         v[3] = pr.T;
-        v[4] = a.Q[3 * a.N + gi];
+        if constexpr (TWO) {
+            v[5] = pr.v;
+            v[6] = a.Q[3 * a.N + gi];
+        } else {
+            v[1] = pr.v;
+            v[4] = a.Q[3 * a.N + gi];
+        }
 #pragma unroll
-        for (int m = 0; m < NMX; m++) v[5 + m] = a.G[msrc(m) * a.N + gi];
+        for (int m = 0; m < NMX; m++) v[mslot(m)] = a.G[msrc(m) * a.N + gi];
     }
     // The mixed-derivative terms, first (the velocities are still staged).
     template <int M, int P, bool NP, bool AL>
@@ -85,7 +102,7 @@ struct PassGrad {
             const int comp = mdst(m);
             double f[M], w[M + 2 * HALO], r[M];
 #pragma unroll
-            for (int k = 0; k < M; k++) f[k] = L.ld(5 + m, k);
+            for (int k = 0; k < M; k++) f[k] = L.ld(mslot(m), k);
             window<M>(f, L, w);
             op_d1<M>(w, a.C, a.K1, a.tab->op[0], L, r);
 #pragma unroll
@@ -95,6 +112,19 @@ struct PassGrad {
                 out[4][k] += L.ld(comp, k) * t;
             }
         }
+    }
+    // TWO: the mixed terms, while their inputs hold the slots of v and rho.
+    template <int M, int P, bool NP, bool AL>
+    DEV static void compute_a(const Geo &, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&out)[5][M])
+    {
+        double mu[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            mu[k] = visc(a.ph, L.ld(3, k));
+#pragma unroll
+            for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+        }
+        compute_mix<M, P, NP>(a, L, mu, out);
     }
     // Accumulate into row i of out (one of the rows in MASK) with static
     // register indices: every row in MASK takes w_r v, w_r = 1 for row i and
@@ -123,10 +153,12 @@ struct PassGrad {
         for (int k = 0; k < M; k++) {
             mu[k] = visc(ph, L.ld(3, k));
             rud[k] = -0.5 * L.ld(4, k) * L.ld(d, k);
+            if constexpr (!TWO) {
 #pragma unroll
-            for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+                for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+            }
         }
-        if constexpr (NMX > 0) compute_mix<M, P, NP>(a, L, mu, out);
+        if constexpr (NMX > 0 && !TWO) compute_mix<M, P, NP>(a, L, mu, out);
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
         for (int it = 0; it < 4; it++) {
@@ -213,15 +245,17 @@ struct PassGrad {
         const int d = g.dir;
         const double pfac = ph.gm1 * ph.ginv;
         double mu[M];
+        if constexpr (!TWO) {
 #pragma unroll
-        for (int k = 0; k < M; k++) {
+            for (int k = 0; k < M; k++) {
 #pragma unroll
-            for (int i = 0; i < 5; i++) out[i][k] = 0.0;
-        }
-        if constexpr (NMX > 0) {
+                for (int i = 0; i < 5; i++) out[i][k] = 0.0;
+            }
+            if constexpr (NMX > 0) {
 #pragma unroll
-            for (int k = 0; k < M; k++) mu[k] = visc(ph, L.ld(3, k));
-            compute_mix<M, P, NP>(a, L, mu, out);
+                for (int k = 0; k < M; k++) mu[k] = visc(ph, L.ld(3, k));
+                compute_mix<M, P, NP>(a, L, mu, out);
+            }
         }
         // flux derivatives: momentum (with pressure) 0..2, energy 4
 #pragma unroll 1
@@ -259,9 +293,10 @@ struct PassGrad {
                 out[3][k] = -sk[k];
             }
         }
-        // the viscosity, first needed here (without mixed terms) so that it
-        // is not held through the flux solves
-        if constexpr (NMX == 0) {
+        // the viscosity, first needed here (without mixed terms, or with
+        // them done in compute_a) so that it is not held through the flux
+        // solves
+        if constexpr (NMX == 0 || TWO) {
 #pragma unroll
             for (int k = 0; k < M; k++) mu[k] = visc(ph, L.ld(3, k));
         }
@@ -421,6 +456,8 @@ hipError_t launch_grad(int kind, const DirTab &t, int nt, int dir, int My, int M
         return launch<PassGrad<1, true, false, 0>>(t, nt, dir, My, Mz, Mx, a, s);
     }
     if (kind == K_GRAD && dir == 2) {
+        // 8 points per lane on 256 threads: the lean order within two waves per SIMD
+        if (nt == 256 && t.M == 8 && t.P == 32) return launch_at<PassGrad<2, true, false, 2, true>, 8, 32, 256>(t, dir, My, Mz, Mx, a, s);
         if (nt == 1024) return launch<PassGrad<1, true, false, 2>, false>(t, nt, dir, My, Mz, Mx, a, s);
         return launch<PassGrad<1, false, false, 2>, false>(t, nt, dir, My, Mz, Mx, a, s);
     }
