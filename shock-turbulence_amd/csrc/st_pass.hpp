@@ -70,6 +70,16 @@ struct hold_n { static constexpr int value = 0; };
 template <class T>
 struct hold_n<T, std::void_t<decltype(T::NHOLD)>> { static constexpr int value = T::NHOLD; };
 
+// NLCARRY of a pass (0 when it names none): values per point that load()
+// returns after the staged and held ones and that wait in registers until
+// the late store (store() takes them as its last argument).  The load and
+// store phases visit the same points per thread, so a value the load phase
+// derives at a point need not be read again to store at that point.
+template <class T, class = void>
+struct lcarry_n { static constexpr int value = 0; };
+template <class T>
+struct lcarry_n<T, std::void_t<decltype(T::NLCARRY)>> { static constexpr int value = T::NLCARRY; };
+
 template <class T, class = void>
 struct has_mid { static constexpr bool value = false; };
 template <class T>
@@ -101,11 +111,12 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     static_assert(!RE || NE <= NS, "early outputs must fit in the input slots");
     constexpr int NPF = PASS::NPF;
     constexpr bool AL = lds_add<PASS, M>::value;
-    constexpr int NH = hold_n<PASS>::value;
+    constexpr int NH = hold_n<PASS>::value, NLC = lcarry_n<PASS>::value;
     static_assert(NPF == 0 || PASS::LOAD_UNROLL >= M, "prefetching needs the load loop unrolled");
-    static_assert(NH == 0 || PASS::LOAD_UNROLL >= M, "holding needs the load loop unrolled");
+    static_assert(NH + NLC == 0 || PASS::LOAD_UNROLL >= M, "holding needs the load loop unrolled");
     double pf[M][NPF > 0 ? NPF : 1];
     double hold[M][NH > 0 ? NH : 1];
+    double lc[M][NLC > 0 ? NLC : 1];
     constexpr int NCR = carry_n<PASS>::value;
     static_assert(NCR == 0 || (RE && NO > 0), "a carry goes from store_early to store");
     double cr[M][NCR > 0 ? NCR : 1];
@@ -122,7 +133,7 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
     for (int r = 0; r < M; r++) {
         int b, s;
         const long gi = W.at(g, r, b, s);
-        double v[NS + NH > 0 ? NS + NH : 1];
+        double v[NS + NH + NLC > 0 ? NS + NH + NLC : 1];
         PASS::load(g, a, gi, v);
         if constexpr (NPF > 0) PASS::prefetch(g, a, gi, pf[r]);
         const int k = s % M, c = s / M;
@@ -131,6 +142,8 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
         for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
 #pragma unroll
         for (int j = 0; j < NH; j++) hold[r][j] = v[NS + j];
+#pragma unroll
+        for (int j = 0; j < NLC; j++) lc[r][j] = v[NS + NH + j];
     }
     __syncthreads();
 
@@ -250,7 +263,9 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             for (int q = 0; q < NK; q++) v[NV + q] = lds[(NS - NK + q) * ss + li];
 #pragma unroll
             for (int q = 0; q < NPF; q++) v[NV + NK + q] = pf[r][q];
-            if constexpr (NSF > 0) PASS::store(g, a, gi, v, cr[r], sf[j]);
+            static_assert(NLC == 0 || (NSF > 0 && NCR > 0), "a load carry goes with store-phase fetches and a carry");
+            if constexpr (NLC > 0) PASS::store(g, a, gi, v, cr[r], sf[j], lc[r]);
+            else if constexpr (NSF > 0) PASS::store(g, a, gi, v, cr[r], sf[j]);
             else if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
             else PASS::store(g, a, gi, v);
         }

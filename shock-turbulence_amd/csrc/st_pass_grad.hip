@@ -49,6 +49,9 @@ struct PassGrad {
     static_assert(!TWO || NMX == 2, "two-round staging is the z sweep's");
     static constexpr int NSLOT = TWO ? 5 : 5 + NMX, NOUT = 5, NEARLY = 5;
     static constexpr int NHOLD = TWO ? 2 : 0;
+    // CROSS: u, v, w and T of each point go from the load phase to the late
+    // store in registers (the store assembles the right-hand side from them)
+    static constexpr int NLCARRY = CROSS ? 4 : 0;
     HD static constexpr int hold_slot(int j) { return j == 0 ? 1 : 4; }
     static constexpr bool REUSE = true;
     static constexpr int NCARRY = CROSS ? 4 : 0;
@@ -65,10 +68,10 @@ struct PassGrad {
     // conflicts in the staging
     HD static constexpr bool lds_add(int m) { return NMX != 2 || m >= 8; }
     // CROSS: the store phases' global reads (sfetch_early: the z and x
-    // partials of the Laplacian of e; sfetch: the state, four x and z
-    // derivatives, and the z and x partial sums) go out for all the
-    // thread's points (two at a time) before the first store
-    static constexpr int NSFE = CROSS ? 2 : 0, NSF = CROSS ? 19 : 0, SF_BATCH = 2;
+    // partials of the Laplacian of e; sfetch: four x and z derivatives, and
+    // the z and x partial sums) go out for all the thread's points (two at a
+    // time) before the first store
+    static constexpr int NSFE = CROSS ? 2 : 0, NSF = CROSS ? 14 : 0, SF_BATCH = 2;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
     // G index of mixed input m: y: gxx=0, gyx=1, gyz=7, gzz=8; z: gxx=0, gzx=2
     HD static constexpr int msrc(int m) { return NMX == 4 ? (m == 0 ? 0 : m == 1 ? 1 : m == 2 ? 7 : 8) : (m == 0 ? 0 : 2); }
@@ -76,12 +79,18 @@ struct PassGrad {
     HD static constexpr int mdst(int m) { return NMX == 4 ? (m == 1 ? 0 : m == 2 ? 2 : 1) : (m == 0 ? 2 : 0); }
     // slot of mixed input m
     HD static constexpr int mslot(int m) { return TWO ? (m == 0 ? 1 : 4) : 5 + m; }
-    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT + NHOLD])
+    DEV static void load(const Geo &, const PassArgs &a, long gi, double (&v)[NSLOT + NHOLD + NLCARRY])
     {
         const Prim pr = prim(a.ph, a.Q[gi], a.Q[a.N + gi], a.Q[2 * a.N + gi], a.Q[3 * a.N + gi], a.Q[4 * a.N + gi]);
         v[0] = pr.u;
         v[2] = pr.w;
         v[3] = pr.T;
+        if constexpr (CROSS) {
+            v[NSLOT] = pr.u;
+            v[NSLOT + 1] = pr.v;
+            v[NSLOT + 2] = pr.w;
+            v[NSLOT + 3] = pr.T;
+        }
         if constexpr (TWO) {
             v[5] = pr.v;
             v[6] = a.Q[3 * a.N + gi];
@@ -385,19 +394,17 @@ struct PassGrad {
         for (int c = 0; c < 4; c++) cr[c] = v[c];
         if (a.L2e) a.L2e[gi] = (v[4] + sf[0]) + sf[1];
     }
-    // the store phase's reads: Q (5), du/dz, dw/dx, dT/dx, dT/dz, the z
-    // partials (5), the x partials (5)
+    // the store phase's reads: du/dz, dw/dx, dT/dx, dT/dz, the z partials
+    // (5), the x partials (5)
     DEV static void sfetch(const Geo &, const PassArgs &a, long gi, double *sf)
     {
         const long N = a.N;
+        sf[0] = a.G[2 * N + gi];
+        sf[1] = a.G[6 * N + gi];
+        sf[2] = a.G[9 * N + gi];
+        sf[3] = a.G[11 * N + gi];
 #pragma unroll
-        for (int c = 0; c < 5; c++) sf[c] = a.Q[c * N + gi];
-        sf[5] = a.G[2 * N + gi];
-        sf[6] = a.G[6 * N + gi];
-        sf[7] = a.G[9 * N + gi];
-        sf[8] = a.G[11 * N + gi];
-#pragma unroll
-        for (int c = 0; c < 10; c++) sf[9 + c] = a.dQp[c * N + gi];
+        for (int c = 0; c < 10; c++) sf[4 + c] = a.dQp[c * N + gi];
     }
     // CROSS: viscous terms built from products of first derivatives
     // (src/rhsv_periodic_sf.f:1128-1154 without the mixed and own-direction
@@ -407,25 +414,25 @@ struct PassGrad {
     // constant-coefficient src/hyper_routines.f); |S| and C_beta only with
     // the artificial transport on (L2e set).
     DEV static void store(const Geo &, const PassArgs &a, long gi, const double *v, const double *cr,
-                          const double *sf)
+                          const double *sf, const double *lc)
     {
         const long N = a.N;
         const Phys &ph = a.ph;
-        const Prim pr = prim(ph, sf[0], sf[1], sf[2], sf[3], sf[4]);
+        struct { double u, v, w, T; } pr = {lc[0], lc[1], lc[2], lc[3]};
         static_assert(!CROSS || (NMX == 4 && msrc(0) == 0 && msrc(1) == 1 && msrc(2) == 7 && msrc(3) == 8),
                       "the kept slots are du/dx, dv/dx, dv/dz, dw/dz");
-        const double gxx = v[5], gyx = v[6], gzx = sf[5];
+        const double gxx = v[5], gyx = v[6], gzx = sf[0];
         const double gxy = cr[0], gyy = cr[1], gzy = cr[2];
-        const double gxz = sf[6], gyz = v[7], gzz = v[8];
+        const double gxz = sf[1], gyz = v[7], gzz = v[8];
         const double mu = visc(ph, pr.T), mup = dvisc(ph, pr.T, mu);
-        const double tx = mup * sf[7], ty = mup * cr[3], tz = mup * sf[8];
+        const double tx = mup * sf[2], ty = mup * cr[3], tz = mup * sf[3];
         const double f1 = ph.bulk - 2.0 / 3.0;
         const double dil = gxx + gyy + gzz;
         const double t1 = gyx + gxy, t2 = gzx + gxz, t3 = gzy + gyz;
         const double fX = tx * (2.0 * gxx + f1 * dil) + ty * t1 + tz * t2;
         const double fY = ty * (2.0 * gyy + f1 * dil) + tx * t1 + tz * t3;
         const double fZ = tz * (2.0 * gzz + f1 * dil) + tx * t2 + ty * t3;
-        const double *Pz = sf + 9, *Px = sf + 14;
+        const double *Pz = sf + 4, *Px = sf + 9;
         a.dQ[gi] = ((v[0] + Pz[0]) + Px[0]) + fX;
         a.dQ[N + gi] = ((v[1] + Pz[1]) + Px[1]) + fY;
         a.dQ[2 * N + gi] = ((v[2] + Pz[2]) + Px[2]) + fZ;
