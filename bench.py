@@ -46,11 +46,15 @@ def kernel_symbol(kind, d, narrow, nt, n):
     P = n // M
     if narrow and 2 * P <= 64:
         M, P = M // 2, 2 * P
-    if kind == "grad":
-        mx = 2 if d == "z" else 0
-        st = "PassGrad<1, %s, false, %d>" % ("true" if nt == 1024 else "false", mx)
+    if kind == "grad" and d == "x":
+        st = "PassGrad<1, true, false, 0, false>"
+    elif kind == "grad":
+        if nt == 256 and M == 8 and P == 32:
+            st = "PassGrad<2, true, false, 2, true>"
+        else:
+            st = "PassGrad<1, %s, false, 2, false>" % ("true" if nt == 1024 else "false")
     elif kind == "grad_cross":
-        st = "PassGrad<1, true, true, 4>" if nt == 1024 else "PassGrad<2, false, true, 4>"
+        st = "PassGrad<1, true, true, 4, false>" if nt == 1024 else "PassGrad<2, false, true, 4, false>"
     elif kind == "lap_se":
         st = "PassLapSE<%s>" % ("false" if d == "x" else "true")
     else:

@@ -194,13 +194,13 @@ double kind_fields(int kind, int dir, const st::PassArgs &a)
     case st::K_RKF: return 15 + 5 + 5;
     case st::K_FQ: return 10;
     case st::K_FQ_FIX: return 10;
-    // Q, G of y (write), the 4 inputs of the mixed terms, G of x and z (8),
-    // the z and x partials, dQ; with the
-    // artificial transport the Laplacian of e (2 partials + the sum), |S|
-    // and C_beta
     // the divergence inputs (Hc, 8 G, Q), dQ, QW; QW and Q written
     case st::K_HDIV_RKF: return 3 + 8 + 5 + 5 + 5 + 5 + 5;
-    case st::K_GRAD_CROSS: return 5 + 4 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
+    // Q, G of y (written), 8 G of x and z (the four inputs of the mixed
+    // terms and four more), the z and x partials, dQ (written); with the
+    // artificial transport the Laplacian of e (2 partials in, the sum out),
+    // |S| and C_beta
+    case st::K_GRAD_CROSS: return 5 + 4 + 8 + 10 + 5 + (a.L2e ? 3 + 1 + (a.Cb ? 1 : 0) : 0);
     default: return 0;
     }
 }

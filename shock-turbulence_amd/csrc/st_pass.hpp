@@ -72,9 +72,10 @@ struct hold_n<T, std::void_t<decltype(T::NHOLD)>> { static constexpr int value =
 
 // NLCARRY of a pass (0 when it names none): values per point that load()
 // returns after the staged and held ones and that wait in registers until
-// the late store (store() takes them as its last argument).  The load and
-// store phases visit the same points per thread, so a value the load phase
-// derives at a point need not be read again to store at that point.
+// the memory-order phase that uses them: mid() for a pass with MID, else
+// the late store (either takes them as its last argument).  These phases
+// visit the same points per thread as the load, so a value the load phase
+// reads or derives at a point need not be read again there.
 template <class T, class = void>
 struct lcarry_n { static constexpr int value = 0; };
 template <class T>
@@ -183,7 +184,8 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             double v[NS];
 #pragma unroll
             for (int q = 0; q < NS; q++) v[q] = lds[q * ss + li];
-            PASS::mid(g, a, gi, s, v);
+            if constexpr (NLC > 0) PASS::mid(g, a, gi, s, v, lc[r]);
+            else PASS::mid(g, a, gi, s, v);
 #pragma unroll
             for (int q = 0; q < NS; q++) lds[q * ss + li] = v[q];
         }
@@ -263,8 +265,9 @@ DEV void pass_body(const Geo &g, const PassArgs &a0, long tile)
             for (int q = 0; q < NK; q++) v[NV + q] = lds[(NS - NK + q) * ss + li];
 #pragma unroll
             for (int q = 0; q < NPF; q++) v[NV + NK + q] = pf[r][q];
-            static_assert(NLC == 0 || (NSF > 0 && NCR > 0), "a load carry goes with store-phase fetches and a carry");
-            if constexpr (NLC > 0) PASS::store(g, a, gi, v, cr[r], sf[j], lc[r]);
+            constexpr bool LCS = NLC > 0 && !has_mid<PASS>::value;
+            static_assert(!LCS || (NSF > 0 && NCR > 0), "a load carry to the store goes with store-phase fetches and a carry");
+            if constexpr (LCS) PASS::store(g, a, gi, v, cr[r], sf[j], lc[r]);
             else if constexpr (NSF > 0) PASS::store(g, a, gi, v, cr[r], sf[j]);
             else if constexpr (NCR > 0) PASS::store(g, a, gi, v, cr[r]);
             else PASS::store(g, a, gi, v);

@@ -27,21 +27,26 @@ struct PassHdivRkf {
     static constexpr int MIN_WAVES = 1;
     static constexpr int LOAD_UNROLL = 8;
     static constexpr int NPF = 0;
+    // the momenta and density that the fluxes read go on to the update in
+    // registers (mid reads only the energy of Q)
+    static constexpr int NLCARRY = 4;
     DEV static void prefetch(const Geo &, const PassArgs &, long, double *) {}
-    DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[5])
+    DEV static void load(const Geo &g, const PassArgs &a, long gi, double (&v)[5 + NLCARRY])
     {
         double t[4];
         PassHyperDiv::load(g, a, gi, t);
 #pragma unroll
         for (int c = 0; c < 4; c++) v[c] = t[c];
         v[4] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[5 + c] = a.Q[c * a.N + gi];
     }
     template <int M, int P, bool NP, bool AL>
     DEV static void compute(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L, double (&o)[1][M])
     {
         PassHyperDiv::compute<M, P, NP>(g, a, L, o);
     }
-    DEV static void mid(const Geo &g, const PassArgs &a, long gi, int ix, double (&v)[5])
+    DEV static void mid(const Geo &g, const PassArgs &a, long gi, int ix, double (&v)[5], const double *lq)
     {
         const long N = a.N;
         double dq[5];
@@ -58,7 +63,7 @@ struct PassHdivRkf {
 #pragma unroll
         for (int c = 0; c < 5; c++) {
             const long o = c * N + gi;
-            const double q = a.Q[o];
+            const double q = c < 4 ? lq[c] : a.Q[o];
             if (sg != 0.0) dq[c] = fma(-sg, q - ref[c], dq[c]);
             const double qw = fma(a.dt, dq[c], a.rk_a * a.QW[o]);   // as PassFilterQ<0>
             a.QW[o] = qw;
