@@ -759,7 +759,8 @@ int stage_loop(st_ctx *c, double dt, double *time, int *step)
         const double *ra, *rb;
         const int ns = rk_loop(c->p.rk_scheme, loop, &ra, &rb);
         if (!ns) break;
-        TRY(fill(c, c->QW, 5, base, 0.0));                   // QW = 0 (:339, :376)
+        // QW = 0 at the loop start (:339, :376): the first stage's alpha is 0
+        // and the update then does not read QW
         for (int k = 0; k < ns; k++) TRY(seq_stage(c, base, ra[k], rb[k], dt));
         if (time) *time = *time + dt;                        // :372, :419
     }
@@ -793,8 +794,7 @@ int batch_step(st_ctx *c)
         const double *ra, *rb;
         const int ns = rk_loop(c->p.rk_scheme, loop, &ra, &rb);
         if (!ns) break;
-        TRY(fill(c, c->QW, 5, a, 0.0));
-        for (int k = 0; k < ns; k++) TRY(seq_stage(c, a, ra[k], rb[k], 0.0));
+        for (int k = 0; k < ns; k++) TRY(seq_stage(c, a, ra[k], rb[k], 0.0));   // (QW: as stage_loop)
         const int last = loop == 1 || !rk_loop(c->p.rk_scheme, 1, &ra, &rb);
         err = st::run_clock(c->ns, c->s_run, c->s_dt, c->s_time, c->s_step, last, c->s);
         if (err != hipSuccess) return fail(c, -4, "batch clock: %s", hipGetErrorString(err));
@@ -1234,12 +1234,6 @@ int group_step(st_group *G)
         const double *ra, *rb;
         const int ns = rk_loop(p.rk_scheme, loop, &ra, &rb);
         if (!ns) break;
-        for (int c = c0; c < 4; c++) {
-            st_ctx *m = G->m[c];
-            if (!m || !m->nact) continue;
-            err = st::run_fill_slots(m->QW, 5, m->N, m->ni, a[c].nslot, a[c].slots, a[c].act, 0.0, G->s);
-            if (err != hipSuccess) return st::set_call_error(-4, "group fill: %s", hipGetErrorString(err));
-        }
         for (int k = 0; k < ns; k++) {
             for (int c = c0; c < 4; c++) {
                 st_ctx *m = G->m[c];

@@ -26,8 +26,10 @@ struct PassFilterQ {
             const long o = c * a.N + gi;
             if (KIND == 0) {
                 // contractions written out, so that the fused form of this
-                // update (PassHdivRkf) rounds alike
-                const double qw = fma(a.dt, a.dQ[o], a.rk_a * a.QW[o]);
+                // update (PassHdivRkf) rounds alike; the first stage of a
+                // loop (alpha = 0) does not read QW, so the loop needs no
+                // clearing of it
+                const double qw = a.rk_a == 0.0 ? fma(a.dt, a.dQ[o], 0.0) : fma(a.dt, a.dQ[o], a.rk_a * a.QW[o]);
                 a.QW[o] = qw;
                 v[c] = fma(a.rk_b, qw, a.Q[o]);
             } else {

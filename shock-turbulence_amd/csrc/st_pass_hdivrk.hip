@@ -65,7 +65,8 @@ struct PassHdivRkf {
             const long o = c * N + gi;
             const double q = c < 4 ? lq[c] : a.Q[o];
             if (sg != 0.0) dq[c] = fma(-sg, q - ref[c], dq[c]);
-            const double qw = fma(a.dt, dq[c], a.rk_a * a.QW[o]);   // as PassFilterQ<0>
+            // as PassFilterQ<0> (alpha = 0: QW not read)
+            const double qw = a.rk_a == 0.0 ? fma(a.dt, dq[c], 0.0) : fma(a.dt, dq[c], a.rk_a * a.QW[o]);
             a.QW[o] = qw;
             v[c] = fma(a.rk_b, qw, q);
         }
