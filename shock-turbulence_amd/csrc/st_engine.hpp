@@ -668,6 +668,20 @@ DEV Prim prim(const Phys &ph, double q1, double q2, double q3, double q4, double
     return r;
 }
 
+// Fix_Pressure at a point (src/lddrk4_periodic_sf.f:436-473): the energy
+// of the state (q1..q5) with P = (gamma-1)(q5 - |q1..3|^2 / 2 q4) clamped to
+// 0.71463 where negative.  Without contraction, so that every pass that
+// applies it rounds alike.
+DEV double fixed_energy(const Phys &ph, double q1, double q2, double q3, double q4, double q5)
+{
+#pragma clang fp contract(off)
+    const double vs = 1.0 / q4;
+    const double ke = q1 * q1 + q2 * q2 + q3 * q3;
+    double p = ph.gm1 * (q5 - 0.5 * vs * ke);
+    if (p < 0.0) p = 0.71463;
+    return ph.gm1_inv * p + 0.5 * vs * ke;
+}
+
 // Compute_CFL's per-point spectral radii (src/lddrk4_periodic_sf.f:29-49):
 // sum over directions of (c + |u_d|) k / Delta_d with k = 2.478 (w1, the
 // stability bound) and 2.199 (w2, the accuracy bound), c = sqrt(gamma (gamma-1) e).

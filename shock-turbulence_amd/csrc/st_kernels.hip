@@ -207,12 +207,7 @@ __global__ void __launch_bounds__(256) k_fixp(PassArgs a, double *Qx)
     const long N = a.N;
     const Phys &ph = a.ph;
     for (long gi = blockIdx.x * (long)blockDim.x + threadIdx.x; gi < a.ni; gi += (long)gridDim.x * blockDim.x) {
-        const double q1 = Qx[gi], q2 = Qx[N + gi], q3 = Qx[2 * N + gi], q4 = Qx[3 * N + gi], q5 = Qx[4 * N + gi];
-        const double vs = 1.0 / q4;
-        const double ke = q1 * q1 + q2 * q2 + q3 * q3;
-        double p = ph.gm1 * (q5 - 0.5 * vs * ke);
-        if (p < 0.0) p = 0.71463;
-        Qx[4 * N + gi] = ph.gm1_inv * p + 0.5 * vs * ke;
+        Qx[4 * N + gi] = fixed_energy(ph, Qx[gi], Qx[N + gi], Qx[2 * N + gi], Qx[3 * N + gi], Qx[4 * N + gi]);
     }
 }
 

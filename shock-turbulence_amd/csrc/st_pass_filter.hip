@@ -57,16 +57,11 @@ struct PassFilterQ {
     {
         const long N = a.N;
         if (KIND == 2 && a.fix) {
-            const Phys &ph = a.ph;
-            const double vs = 1.0 / v[3];
-            const double ke = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            double p = ph.gm1 * (v[4] - 0.5 * vs * ke);
-            if (p < 0.0) p = 0.71463;
             a.Q[gi] = v[0];
             a.Q[N + gi] = v[1];
             a.Q[2 * N + gi] = v[2];
             a.Q[3 * N + gi] = v[3];
-            a.Q[4 * N + gi] = ph.gm1_inv * p + 0.5 * vs * ke;
+            a.Q[4 * N + gi] = fixed_energy(a.ph, v[0], v[1], v[2], v[3], v[4]);
             return;
         }
 #pragma unroll
