@@ -22,9 +22,10 @@ namespace st {
 // ---------------------------------------------------------------------
 // MW: occupancy target (waves per SIMD) handed to the register allocator;
 // 2 for the y pass, whose nine staged fields allow two 256-thread
-// workgroups per CU anyway; the strided x and z lines run faster with the
-// compiler's own choice.  LEAN selects the operator order that keeps
-// fewer values live (for 1024-thread tiles, which allow 128 registers).
+// workgroups per CU anyway, and for the z pass at 8 points per lane (two
+// 8-line workgroups per CU); otherwise the compiler's own choice.  LEAN
+// selects the operator order that keeps fewer values live (the x pass at
+// every tile width, the z pass at 1024 threads and at 8 points per lane).
 // CROSS (y lines, after the x and z passes have left their partial sums):
 // the store phase also assembles the right-hand side of the first-derivative
 // part -- the y terms, the z and x partials and the viscous terms built from
