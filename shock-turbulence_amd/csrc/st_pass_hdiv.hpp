@@ -33,6 +33,11 @@ struct PassHyperDiv {
         fluxes(g, a, gi, a.Hc[gi], a.Hc[N + gi], a.Hc[2 * N + gi], v);
     }
     // the staged fluxes of direction g.dir from the coefficients at the point
+    // (SEL: the direction's gradient entries chosen by index, so that the
+    // loads of a point issue together, which the load phases of hyper_div
+    // and hdiv_rkf gain from (2-3%); the z pass of gauss_hdiv computes them
+    // in its mid phase, where the branch on the direction measured faster)
+    template <bool SEL = true>
     DEV static void fluxes(const Geo &g, const PassArgs &a, long gi, double beta, double mu, double kap,
                            double (&v)[4])
     {
@@ -45,7 +50,21 @@ struct PassHyperDiv {
         const double tt = 2.0 / 3.0;
         const double bulkterm = (beta - tt * mu) * dil;
         double tx, ty, tz;
-        if (g.dir == 0) {
+        if constexpr (SEL) {
+            // the two off-diagonal stresses of direction d from the pairs
+            // (du_i/dx_j, du_j/dx_i) with the other two directions: x-y
+            // (3, 1), x-z (6, 2), y-z (7, 5) (d is the same on every lane)
+            const int d = g.dir;
+            const int a1 = d == 2 ? 6 : 3, a2 = d == 2 ? 2 : 1;
+            const int b1 = d == 0 ? 6 : 7, b2 = d == 0 ? 2 : 5;
+            const double sa = mu * (G[a1 * N + gi] + G[a2 * N + gi]);
+            const double sb = mu * (G[b1 * N + gi] + G[b2 * N + gi]);
+            const double gdd = d == 0 ? gxx : d == 1 ? gyy : gzz;
+            const double diag = 2.0 * mu * gdd + bulkterm;
+            tx = d == 0 ? diag : sa;
+            ty = d == 1 ? diag : d == 0 ? sa : sb;
+            tz = d == 2 ? diag : sb;
+        } else if (g.dir == 0) {
             tx = 2.0 * mu * gxx + bulkterm;
             ty = mu * (G[3 * N + gi] + G[1 * N + gi]);
             tz = mu * (G[6 * N + gi] + G[2 * N + gi]);

@@ -220,7 +220,7 @@ struct PassGaussHdiv {
         a.Hc[gi] = beta;
         a.Hc[N + gi] = mu;
         a.Hc[2 * N + gi] = kap;
-        PassHyperDiv::fluxes(g, a, gi, beta, mu, kap, v);
+        PassHyperDiv::fluxes<false>(g, a, gi, beta, mu, kap, v);
     }
     template <int M, int P, bool NP, bool AL>
     DEV static void compute2(const Geo &g, const PassArgs &a, const Lane<M, P, NP, AL> &L)
